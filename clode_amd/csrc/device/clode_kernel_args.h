@@ -1,0 +1,64 @@
+// Kernel argument block shared by the host runtime (g++/clang++, x86-64) and
+// the hipRTC device programs (gfx950).  Passed by value as the single kernel
+// argument; every field has the same size and alignment on both targets.
+// Scalars are carried in double and rounded to the program's working
+// precision inside the kernel, which is the same round-to-nearest conversion
+// the reference applies on the host before copying (clode/cpp/CLODE.cpp:405-417,
+// CLODEfeatures.cpp:125-141).
+#ifndef CLODE_KERNEL_ARGS_H
+#define CLODE_KERNEL_ARGS_H
+
+struct ClodeArgs {
+    // time span (clode/cpp/transient.cl:10)
+    double t0;
+    double t1;
+    // SolverParams (clode/cpp/clODE_struct_defs.cl:11-20)
+    double sp_dt;
+    double sp_dtmax;
+    double sp_abstol;
+    double sp_reltol;
+    unsigned int sp_max_steps;
+    unsigned int sp_max_store;
+    unsigned int sp_nout;
+    unsigned int nPts;
+    // ObserverParams (clode/cpp/observers.cl:25-46)
+    unsigned int op_eVarIx;
+    unsigned int op_fVarIx;
+    unsigned int op_maxEventCount;
+    unsigned int op_maxEventTimestamps;
+    double op_minXamp;
+    double op_minIMI;
+    double op_nHoodRadius;
+    double op_xUpThresh;
+    double op_xDownThresh;
+    double op_dxUpThresh;
+    double op_dxDownThresh;
+    double op_eps_dx;
+    // member arrays, structure of arrays: element [j][i] at j*nPts + i
+    void *x0;            // [nVar][nPts]   state in  (x store type)
+    void *pars;          // [nPar][nPts]   realtype
+    void *xf;            // [nVar][nPts]   state out (x store type)
+    unsigned long long *rng;  // [2][nPts]  xoroshiro128+ words
+    void *dt;            // [nPts]          step size in/out
+    void *tf;            // [nPts]          final time
+    void *odr;           // [nReal][nPts]   observer reals
+    unsigned int *odu;   // [nUint][nPts]   observer unsigned ints
+    void *F;             // [nFeat][nPts]   features
+    void *tr_t;          // [maxStore][nPts]
+    void *tr_x;          // [maxStore][nVar][nPts]
+    void *tr_dx;         // [maxStore][nVar][nPts]
+    void *tr_aux;        // [maxStore][nAux][nPts]
+    int *nStored;        // [nPts]
+    // persistent / batched launches
+    unsigned int *work_counter;  // next member to claim (persistent kernel)
+    void *tspan_m;               // optional per-member [2][nPts] time spans (double)
+    unsigned int *status;        // optional per-member status word
+    unsigned int member_offset;  // first member handled by this launch
+    unsigned int n_launch;       // members handled by this launch
+};
+
+#ifdef __cplusplus
+static_assert(sizeof(void *) == 8, "64-bit targets only");
+#endif
+
+#endif  // CLODE_KERNEL_ARGS_H

@@ -1,0 +1,540 @@
+// Time steppers, one ensemble member per lane, all state in registers.
+//
+// Registry (name -> CLODE_STEPPER id), as clode/cpp/steppers.cl:26-43:
+//   euler=1  heun=2  rk4=3  bs23=4  dopri5=5  seuler=6 (Euler-Maruyama)
+//
+// Plain modes (CLODE_PREC 1 = single, 2 = double) restate the reference's
+// arithmetic expression by expression so that, compiled with
+// -ffp-contract=on, every multiply-add is fused exactly where the reference's
+// OpenCL C would be:
+//   fixed-step wrapper  clode/cpp/steppers/fixed_explicit_step.clh:9-36
+//   Euler               clode/cpp/steppers/fixed_explicit_Euler.clh:5-17
+//   Heun                clode/cpp/steppers/fixed_explicit_Trapezoidal.clh:5-22
+//   RK4                 clode/cpp/steppers/fixed_explicit_RK4.clh:5-39
+//   adaptive wrapper    clode/cpp/steppers/adaptive_explicit_step.clh:4-80
+//   Bogacki-Shampine    clode/cpp/steppers/adaptive_bs23.clh:3-62
+//   Dormand-Prince      clode/cpp/steppers/adaptive_dp45.clh:4-110
+//   seuler = Euler + fresh Wiener increments after each update
+//                       (clode/cpp/steppers.cl:54-57)
+// Tableau constants are written as quotients of realtype literals so that
+// single precision folds them to the correctly rounded float quotient.
+//
+// fp32-compensated mode (CLODE_PREC 3) keeps the same tableaux but carries
+// time and the step size in double, adds each state increment with Kahan
+// compensation (x, c), evaluates the embedded error combination in double
+// (full-rate FP64 FMA on CDNA) and drives the controller from that double
+// error norm; stage algebra and the right-hand side run in float.
+#ifndef CLODE_STEPPERS_H
+#define CLODE_STEPPERS_H
+
+#if CLODE_STEPPER == 1 || CLODE_STEPPER == 2 || CLODE_STEPPER == 3 || CLODE_STEPPER == 6
+#define CLODE_FIXED_STEP 1
+#elif CLODE_STEPPER == 4 || CLODE_STEPPER == 5
+#define CLODE_ADAPTIVE_STEP 1
+#else
+#error "unknown CLODE_STEPPER"
+#endif
+
+#if CLODE_STEPPER == 6
+#define CLODE_STOCHASTIC 1
+#endif
+
+// Bound on step attempts per accepted step.  The reference's retry loop
+// (adaptive_explicit_step.clh:20-62) ends at hmin; if rounding of t + dt keeps
+// the attempted step just above hmin it would not end, so after this many
+// attempts the step is treated as an hmin failure.
+#define CLODE_MAX_ATTEMPTS 100
+
+#define N_PAR1 (N_PAR > 0 ? N_PAR : 1)
+#define N_AUX1 (N_AUX > 0 ? N_AUX : 1)
+#define N_WIENER1 (N_WIENER > 0 ? N_WIENER : 1)
+
+// Solver parameters in working precision (clode/cpp/clODE_struct_defs.cl:11-20)
+struct SolverParamsR {
+    realtype dt;
+    realtype dtmax;
+    realtype abstol;
+    realtype reltol;
+    unsigned int max_steps;
+    unsigned int max_store;
+    unsigned int nout;
+};
+
+#if CLODE_PREC != 3
+// =====================================================================
+//  Plain single / double precision (reference semantics)
+// =====================================================================
+
+#if CLODE_STEPPER == 1 || CLODE_STEPPER == 6
+CLODE_DEV void do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                       const realtype dt, realtype aux[], const realtype wi[])
+{
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] += dt * k1[k];
+    ti += dt;
+}
+#endif
+
+#if CLODE_STEPPER == 2
+CLODE_DEV void do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                       const realtype dt, realtype aux[], const realtype wi[])
+{
+    realtype th = ti + dt;
+    realtype tmp[N_VAR], k2[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = fma(dt, k1[k], xi[k]);
+    getRHS(th, tmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] += dt * RCONST(0.5) * (k1[k] + k2[k]);
+    ti = th;
+}
+#endif
+
+#if CLODE_STEPPER == 3
+CLODE_DEV void do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                       const realtype dt, realtype aux[], const realtype wi[])
+{
+    realtype tmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR];
+    realtype h2 = dt * RCONST(0.5);
+    realtype th2 = ti + h2;
+    realtype th = ti + dt;
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + h2 * k1[k];
+    getRHS(th2, tmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + h2 * k2[k];
+    getRHS(th2, tmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + dt * k3[k];
+    getRHS(th, tmp, pars, k4, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] += dt * (k1[k] + RCONST(2.0) * k2[k] + RCONST(2.0) * k3[k] + k4[k]) / RCONST(6.0);
+    ti = th;
+}
+#endif
+
+#if CLODE_FIXED_STEP
+// Fixed-step wrapper: dt never changes; k1 is re-evaluated at the new point
+// (after a fresh Wiener draw for seuler) and reused as the next step's k1.
+CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                      const SolverParamsR &sp, realtype &dt, const realtype t1,
+                      realtype aux[], realtype wi[], RngData &rd)
+{
+    do_step(ti, xi, k1, pars, dt, aux, wi);
+#if CLODE_STOCHASTIC
+    for (int j = 0; j < N_WIENER; ++j)
+        wi[j] = rng_normal(rd) / sqrt(dt);
+#endif
+    getRHS(ti, xi, pars, k1, aux, wi);
+    return 0;
+}
+#endif
+
+#if CLODE_STEPPER == 4
+#define CLODE_ERR_ORDER RCONST(2.0)
+#define CLODE_MAX_SHRINK RCONST(0.5)
+#define CLODE_MAX_GROW RCONST(5.0)
+CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                           const realtype dt, realtype aux[], realtype err[], const realtype wi[])
+{
+    const realtype B1 = RCONST(2.0) / RCONST(9.0);
+    const realtype B2 = RCONST(1.0) / RCONST(3.0);
+    const realtype B3 = RCONST(4.0) / RCONST(9.0);
+    const realtype E1 = RCONST(-5.0) / RCONST(72.0);
+    const realtype E2 = RCONST(1.0) / RCONST(12.0);
+    const realtype E3 = RCONST(1.0) / RCONST(9.0);
+    const realtype E4 = RCONST(-1.0) / RCONST(8.0);
+    realtype tNew = ti + dt;
+    realtype newDt = tNew - ti;
+    realtype xtmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * RCONST(0.5) * k1[k];
+    getRHS(ti + newDt * RCONST(0.5), xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * RCONST(0.75) * k2[k];
+    getRHS(ti + newDt * RCONST(0.75), xtmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] = xi[k] + newDt * (B1 * k1[k] + B2 * k2[k] + B3 * k3[k]);
+    getRHS(tNew, xi, pars, k4, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        err[k] = newDt * (E1 * k1[k] + E2 * k2[k] + E3 * k3[k] + E4 * k4[k]);
+        k1[k] = k4[k];
+    }
+    ti = tNew;
+    return newDt;
+}
+#endif
+
+#if CLODE_STEPPER == 5
+#define CLODE_ERR_ORDER RCONST(4.0)
+#define CLODE_MAX_SHRINK RCONST(0.1)
+#define CLODE_MAX_GROW RCONST(5.0)
+CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                           const realtype dt, realtype aux[], realtype err[], const realtype wi[])
+{
+    const realtype A2 = RCONST(1.0) / RCONST(5.0);
+    const realtype A3 = RCONST(3.0) / RCONST(10.0);
+    const realtype A4 = RCONST(4.0) / RCONST(5.0);
+    const realtype A5 = RCONST(8.0) / RCONST(9.0);
+    const realtype B21 = RCONST(1.0) / RCONST(5.0);
+    const realtype B31 = RCONST(3.0) / RCONST(40.0);
+    const realtype B32 = RCONST(9.0) / RCONST(40.0);
+    const realtype B41 = RCONST(44.0) / RCONST(45.0);
+    const realtype B42 = RCONST(-56.0) / RCONST(15.0);
+    const realtype B43 = RCONST(32.0) / RCONST(9.0);
+    const realtype B51 = RCONST(19372.0) / RCONST(6561.0);
+    const realtype B52 = RCONST(-25360.0) / RCONST(2187.0);
+    const realtype B53 = RCONST(64448.0) / RCONST(6561.0);
+    const realtype B54 = RCONST(-212.0) / RCONST(729.0);
+    const realtype B61 = RCONST(9017.0) / RCONST(3168.0);
+    const realtype B62 = RCONST(-355.0) / RCONST(33.0);
+    const realtype B63 = RCONST(46732.0) / RCONST(5247.0);
+    const realtype B64 = RCONST(49.0) / RCONST(176.0);
+    const realtype B65 = RCONST(-5103.0) / RCONST(18656.0);
+    const realtype C1 = RCONST(35.0) / RCONST(384.0);
+    const realtype C3 = RCONST(500.0) / RCONST(1113.0);
+    const realtype C4 = RCONST(125.0) / RCONST(192.0);
+    const realtype C5 = RCONST(-2187.0) / RCONST(6784.0);
+    const realtype C6 = RCONST(11.0) / RCONST(84.0);
+    const realtype E1 = RCONST(71.0) / RCONST(57600.0);
+    const realtype E3 = RCONST(-71.0) / RCONST(16695.0);
+    const realtype E4 = RCONST(71.0) / RCONST(1920.0);
+    const realtype E5 = RCONST(-17253.0) / RCONST(339200.0);
+    const realtype E6 = RCONST(22.0) / RCONST(525.0);
+    const realtype E7 = RCONST(-1.0) / RCONST(40.0);
+    realtype tNew = ti + dt;
+    realtype newDt = tNew - ti;
+    realtype xtmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR], k5[N_VAR], k6[N_VAR], k7[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * (B21 * k1[k]);
+    getRHS(ti + A2 * newDt, xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * (B31 * k1[k] + B32 * k2[k]);
+    getRHS(ti + A3 * newDt, xtmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * (B41 * k1[k] + B42 * k2[k] + B43 * k3[k]);
+    getRHS(ti + A4 * newDt, xtmp, pars, k4, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * (B51 * k1[k] + B52 * k2[k] + B53 * k3[k] + B54 * k4[k]);
+    getRHS(ti + A5 * newDt, xtmp, pars, k5, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * (B61 * k1[k] + B62 * k2[k] + B63 * k3[k] + B64 * k4[k] + B65 * k5[k]);
+    getRHS(ti + newDt, xtmp, pars, k6, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] = xi[k] + newDt * (C1 * k1[k] + C3 * k3[k] + C4 * k4[k] + C5 * k5[k] + C6 * k6[k]);
+    getRHS(tNew, xi, pars, k7, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        err[k] = newDt * (E1 * k1[k] + E3 * k3[k] + E4 * k4[k] + E5 * k5[k] + E6 * k6[k] + E7 * k7[k]);
+        k1[k] = k7[k];
+    }
+    ti = tNew;
+    return newDt;
+}
+#endif
+
+#if CLODE_ADAPTIVE_STEP
+// Error-per-unit-step controller (MATLAB ode23/ode45 style): elementwise
+// relative error against max(|x|, |xnew|, abstol/reltol), infinity norm,
+// first rejection shrinks by max(MAX_SHRINK, 0.8 (rtol/err)^(1/(p+1))),
+// later rejections halve, growth after an unrejected step is capped at
+// MAX_GROW, and the step is clipped to the end of the span and to
+// [hmin, dtmax] with hmin = 16 ulp(t).  Returns -1 (state unchanged, dt=hmin)
+// when a step at hmin still fails; the drivers ignore the flag, as the
+// reference's do.
+CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                      const SolverParamsR &sp, realtype &dt, const realtype t1,
+                      realtype aux[], realtype wi[], RngData &rd)
+{
+    const realtype EXPON = RCONST(1.0) / (CLODE_ERR_ORDER + RCONST(1.0));
+    realtype tNew, normErr, err[N_VAR], newxi[N_VAR], newk1[N_VAR];
+    realtype newDt = dt;
+    realtype threshold = sp.abstol / sp.reltol;
+    realtype hmin = RCONST(16.0) * fabs(fabs(nextafter(ti, RCONST(1.1) * t1)) - ti);
+    bool noFailedSteps = true;
+    int attempts = 0;
+    while (true) {
+        if (++attempts > CLODE_MAX_ATTEMPTS) {
+            dt = hmin;
+            return -1;
+        }
+        tNew = ti;
+        for (int j = 0; j < N_VAR; j++) {
+            newxi[j] = xi[j];
+            newk1[j] = k1[j];
+        }
+        newDt = fmin(fmax(newDt, hmin), sp.dtmax);
+        newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, wi);
+        for (int j = 0; j < N_VAR; j++)
+            err[j] /= fmax(fmax(fabs(xi[j]), fabs(newxi[j])), threshold);
+        normErr = ZERO;
+        for (int j = 0; j < N_VAR; j++)
+            normErr = fmax(fabs(err[j]), normErr);
+        if (normErr > sp.reltol) {
+            if (newDt <= hmin) {
+                dt = hmin;
+                return -1;
+            }
+            if (noFailedSteps) {
+                noFailedSteps = false;
+                newDt *= fmax(CLODE_MAX_SHRINK, RCONST(0.8) * pow(sp.reltol / normErr, EXPON));
+            } else {
+                newDt *= RCONST(0.5);
+            }
+        } else {
+            break;
+        }
+    }
+    if (noFailedSteps)
+        newDt *= fmin(CLODE_MAX_GROW, RCONST(0.8) * pow(sp.reltol / normErr, EXPON));
+    newDt = fmin(newDt, t1 - tNew);
+    newDt = fmin(fmax(newDt, hmin), sp.dtmax);
+    dt = newDt;
+    ti = tNew;
+    for (int j = 0; j < N_VAR; j++) {
+        xi[j] = newxi[j];
+        k1[j] = newk1[j];
+    }
+    return 0;
+}
+#endif
+
+#else  // CLODE_PREC == 3
+// =====================================================================
+//  fp32-compensated: float state + Kahan compensation, double time/dt,
+//  double embedded-error norm
+// =====================================================================
+
+// x + d with Kahan compensation c (c holds the negated lost low part)
+CLODE_DEV void kahan_add(realtype &x, realtype &c, realtype d)
+{
+    realtype y = d - c;
+    realtype t = x + y;
+    c = (t - x) - y;
+    x = t;
+}
+
+#if CLODE_STEPPER == 1 || CLODE_STEPPER == 6
+CLODE_DEV void do_step(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                       const double dt, realtype aux[], const realtype wi[])
+{
+    const realtype h = (realtype)dt;
+    for (int k = 0; k < N_VAR; k++)
+        kahan_add(xi[k], xc[k], h * k1[k]);
+    ti += dt;
+}
+#endif
+
+#if CLODE_STEPPER == 2
+CLODE_DEV void do_step(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                       const double dt, realtype aux[], const realtype wi[])
+{
+    const realtype h = (realtype)dt;
+    double th = ti + dt;
+    realtype tmp[N_VAR], k2[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = fmaf(h, k1[k], xi[k]);
+    getRHS((realtype)th, tmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        kahan_add(xi[k], xc[k], h * 0.5f * (k1[k] + k2[k]));
+    ti = th;
+}
+#endif
+
+#if CLODE_STEPPER == 3
+CLODE_DEV void do_step(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                       const double dt, realtype aux[], const realtype wi[])
+{
+    const realtype h = (realtype)dt;
+    realtype tmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR];
+    realtype h2 = h * 0.5f;
+    realtype th2 = (realtype)(ti + 0.5 * dt);
+    double th = ti + dt;
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + h2 * k1[k];
+    getRHS(th2, tmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + h2 * k2[k];
+    getRHS(th2, tmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        tmp[k] = xi[k] + h * k3[k];
+    getRHS((realtype)th, tmp, pars, k4, aux, wi);
+    const realtype sixth = 1.0f / 6.0f;
+    for (int k = 0; k < N_VAR; k++)
+        kahan_add(xi[k], xc[k], h * sixth * (k1[k] + 2.0f * k2[k] + 2.0f * k3[k] + k4[k]));
+    ti = th;
+}
+#endif
+
+#if CLODE_FIXED_STEP
+CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                      const SolverParamsR &sp, double &dt, const double t1,
+                      realtype aux[], realtype wi[], RngData &rd, const double, const double, const double)
+{
+    do_step(ti, xi, xc, k1, pars, dt, aux, wi);
+#if CLODE_STOCHASTIC
+    const realtype rsq = 1.0f / sqrtf((realtype)dt);
+    for (int j = 0; j < N_WIENER; ++j)
+        wi[j] = rng_normal(rd) * rsq;
+#endif
+    getRHS((realtype)ti, xi, pars, k1, aux, wi);
+    return 0;
+}
+#endif
+
+#if CLODE_STEPPER == 4
+#define CLODE_EXPON (1.0f / 3.0f)
+#define CLODE_MAX_SHRINK 0.5
+#define CLODE_MAX_GROW 5.0
+// returns the step taken; newxi holds the 3rd-order solution without the
+// compensation term; errd the double error estimate
+CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtype pars[],
+                         const double dt, realtype aux[], double errd[], realtype dx[], const realtype wi[])
+{
+    const realtype B1 = 2.0f / 9.0f, B2 = 1.0f / 3.0f, B3 = 4.0f / 9.0f;
+    const double E1 = -5.0 / 72.0, E2 = 1.0 / 12.0, E3 = 1.0 / 9.0, E4 = -1.0 / 8.0;
+    double tNew = ti + dt;
+    double newDt = tNew - ti;
+    const realtype h = (realtype)newDt;
+    realtype xtmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * 0.5f * k1[k];
+    getRHS((realtype)(ti + 0.5 * newDt), xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * 0.75f * k2[k];
+    getRHS((realtype)(ti + 0.75 * newDt), xtmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        dx[k] = h * (B1 * k1[k] + B2 * k2[k] + B3 * k3[k]);
+        xtmp[k] = xi[k] + dx[k];
+    }
+    getRHS((realtype)tNew, xtmp, pars, k4, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        errd[k] = newDt * (E1 * (double)k1[k] + E2 * (double)k2[k] + E3 * (double)k3[k] + E4 * (double)k4[k]);
+        k1[k] = k4[k];
+        xi[k] = xtmp[k];
+    }
+    ti = tNew;
+    return newDt;
+}
+#endif
+
+#if CLODE_STEPPER == 5
+#define CLODE_EXPON 0.2f
+#define CLODE_MAX_SHRINK 0.1
+#define CLODE_MAX_GROW 5.0
+CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtype pars[],
+                         const double dt, realtype aux[], double errd[], realtype dx[], const realtype wi[])
+{
+    const realtype B21 = 1.0f / 5.0f;
+    const realtype B31 = 3.0f / 40.0f, B32 = 9.0f / 40.0f;
+    const realtype B41 = 44.0f / 45.0f, B42 = -56.0f / 15.0f, B43 = 32.0f / 9.0f;
+    const realtype B51 = 19372.0f / 6561.0f, B52 = -25360.0f / 2187.0f, B53 = 64448.0f / 6561.0f,
+                   B54 = -212.0f / 729.0f;
+    const realtype B61 = 9017.0f / 3168.0f, B62 = -355.0f / 33.0f, B63 = 46732.0f / 5247.0f,
+                   B64 = 49.0f / 176.0f, B65 = -5103.0f / 18656.0f;
+    const realtype C1 = 35.0f / 384.0f, C3 = 500.0f / 1113.0f, C4 = 125.0f / 192.0f,
+                   C5 = -2187.0f / 6784.0f, C6 = 11.0f / 84.0f;
+    const double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0,
+                 E5 = -17253.0 / 339200.0, E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+    double tNew = ti + dt;
+    double newDt = tNew - ti;
+    const realtype h = (realtype)newDt;
+    realtype xtmp[N_VAR], k2[N_VAR], k3[N_VAR], k4[N_VAR], k5[N_VAR], k6[N_VAR], k7[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * (B21 * k1[k]);
+    getRHS((realtype)(ti + 0.2 * newDt), xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * (B31 * k1[k] + B32 * k2[k]);
+    getRHS((realtype)(ti + 0.3 * newDt), xtmp, pars, k3, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * (B41 * k1[k] + B42 * k2[k] + B43 * k3[k]);
+    getRHS((realtype)(ti + 0.8 * newDt), xtmp, pars, k4, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * (B51 * k1[k] + B52 * k2[k] + B53 * k3[k] + B54 * k4[k]);
+    getRHS((realtype)(ti + (8.0 / 9.0) * newDt), xtmp, pars, k5, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * (B61 * k1[k] + B62 * k2[k] + B63 * k3[k] + B64 * k4[k] + B65 * k5[k]);
+    getRHS((realtype)tNew, xtmp, pars, k6, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        dx[k] = h * (C1 * k1[k] + C3 * k3[k] + C4 * k4[k] + C5 * k5[k] + C6 * k6[k]);
+        xtmp[k] = xi[k] + dx[k];
+    }
+    getRHS((realtype)tNew, xtmp, pars, k7, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        errd[k] = newDt * (E1 * (double)k1[k] + E3 * (double)k3[k] + E4 * (double)k4[k] +
+                           E5 * (double)k5[k] + E6 * (double)k6[k] + E7 * (double)k7[k]);
+        k1[k] = k7[k];
+        xi[k] = xtmp[k];
+    }
+    ti = tNew;
+    return newDt;
+}
+#endif
+
+#if CLODE_ADAPTIVE_STEP
+// Same controller as the plain modes; the error ratio and its norm are
+// formed in double, the growth factor (rtol/err)^(1/(p+1)) with the
+// hardware log2/exp2 (it only steers the next step size).
+CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                      const SolverParamsR &sp, double &dt, const double t1,
+                      realtype aux[], realtype wi[], RngData &rd, const double reltol, const double threshold,
+                      const double dtmax)
+{
+    double tNew, normErr, errd[N_VAR];
+    realtype newxi[N_VAR], newk1[N_VAR], dx[N_VAR];
+    double newDt = dt;
+    double hmin = 16.0 * fabs(fabs(nextafter(ti, 1.1 * t1)) - ti);
+    bool noFailedSteps = true;
+    int attempts = 0;
+    while (true) {
+        if (++attempts > CLODE_MAX_ATTEMPTS) {
+            dt = hmin;
+            return -1;
+        }
+        tNew = ti;
+        for (int j = 0; j < N_VAR; j++) {
+            newxi[j] = xi[j];
+            newk1[j] = k1[j];
+        }
+        newDt = fmin(fmax(newDt, hmin), dtmax);
+        newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, errd, dx, wi);
+        normErr = 0.0;
+        for (int j = 0; j < N_VAR; j++) {
+            realtype sc = fmaxf(fmaxf(fabsf(xi[j]), fabsf(newxi[j])), (realtype)threshold);
+            normErr = fmax(fabs(errd[j]) * (double)__frcp_rn(sc), normErr);
+        }
+        if (normErr > reltol) {
+            if (newDt <= hmin) {
+                dt = hmin;
+                return -1;
+            }
+            if (noFailedSteps) {
+                noFailedSteps = false;
+                realtype q = (realtype)(reltol / normErr);
+                newDt *= fmax(CLODE_MAX_SHRINK, (double)(0.8f * exp2f(CLODE_EXPON * log2f(q))));
+            } else {
+                newDt *= 0.5;
+            }
+        } else {
+            break;
+        }
+    }
+    if (noFailedSteps) {
+        realtype q = normErr > 0.0 ? (realtype)(reltol / normErr) : 3.0e38f;
+        newDt *= fmin(CLODE_MAX_GROW, (double)(0.8f * exp2f(CLODE_EXPON * log2f(q))));
+    }
+    newDt = fmin(newDt, t1 - tNew);
+    newDt = fmin(fmax(newDt, hmin), dtmax);
+    dt = newDt;
+    ti = tNew;
+    for (int j = 0; j < N_VAR; j++) {
+        // newxi = xi + dx (rounded); redo the addition with compensation
+        kahan_add(xi[j], xc[j], dx[j]);
+        k1[j] = newk1[j];
+    }
+    return 0;
+}
+#endif
+
+#endif  // CLODE_PREC
+
+#endif  // CLODE_STEPPERS_H

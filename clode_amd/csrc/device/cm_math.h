@@ -1,0 +1,253 @@
+// Deterministic elementary functions for the ensemble integrator.
+//
+// The integrator's results depend on exp/log/pow at three places: the user's
+// right-hand side (e.g. Boltzmann gating functions), the adaptive step-size
+// controller (pow of the error ratio) and the polar Gaussian generator (log).
+// Vendor math libraries differ by an ulp or two between the host libm and the
+// GPU's device library, which would make FP64 runs differ in the last bits from
+// a CPU restatement of the same algorithm.  These routines are written with
+// explicit fused multiply-adds, exactly rounded IEEE operations (+,-,*,/,
+// sqrt, rint, ldexp, frexp) and fixed constants only, so they return the same
+// bits wherever they are compiled (hipRTC for gfx950, clang++ for x86-64 with
+// FMA).  Accuracy against glibc is checked in the test-suite (<= 2 ulp for
+// exp/log, <= 16 ulp for pow, the OpenCL C 1.2 bound for pow/pown).
+//
+// Methods:
+//  exp: Cody-Waite argument reduction x = k ln2 + r, |r| <= ln2/2, followed by
+//       the Taylor polynomial of e^r (degree 13 for double, 7 for float) in
+//       Horner form, scaled exactly by 2^k with ldexp.
+//  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f);
+//       log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))) where R is the series
+//       2z/3 + 2z^2/5 + ... of 2 atanh(s)/s - 2 (11 terms double, 4 float).
+//  pow: IEEE-754 special cases, then exp(y log|x|) with the sign of x for odd
+//       integer y.
+//  pown: binary exponentiation (exact operation sequence).
+//
+// CM_FN is defined by the including translation unit (host or device).
+#ifndef CLODE_CM_MATH_H
+#define CLODE_CM_MATH_H
+
+#ifndef CM_FN
+#define CM_FN inline
+#endif
+
+namespace cm {
+
+// ---------------------------------------------------------------- double --
+CM_FN double exp(double x)
+{
+    if (!(x == x))
+        return x;
+    if (x > 709.782712893384)
+        return __builtin_huge_val();
+    if (x < -745.1332191019412)
+        return 0.0;
+    const double log2e = 0x1.71547652b82fep+0;
+    const double ln2hi = 0x1.62e42fefa39efp-1;
+    const double ln2lo = 0x1.abc9e3b39803fp-56;
+    double k = rint(x * log2e);
+    double r = fma(-k, ln2hi, x);
+    r = fma(-k, ln2lo, r);
+    double p = 0x1.6124613a86d09p-33;
+    p = fma(p, r, 0x1.1eed8eff8d898p-29);
+    p = fma(p, r, 0x1.ae64567f544e4p-26);
+    p = fma(p, r, 0x1.27e4fb7789f5cp-22);
+    p = fma(p, r, 0x1.71de3a556c734p-19);
+    p = fma(p, r, 0x1.a01a01a01a01ap-16);
+    p = fma(p, r, 0x1.a01a01a01a01ap-13);
+    p = fma(p, r, 0x1.6c16c16c16c17p-10);
+    p = fma(p, r, 0x1.1111111111111p-7);
+    p = fma(p, r, 0x1.5555555555555p-5);
+    p = fma(p, r, 0x1.5555555555555p-3);
+    p = fma(p, r, 0x1.0000000000000p-1);
+    p = fma(p, r, 1.0);
+    p = fma(p, r, 1.0);
+    return ldexp(p, (int)k);
+}
+
+CM_FN double log(double x)
+{
+    if (!(x > 0.0)) {
+        if (x == 0.0)
+            return -__builtin_huge_val();
+        return __builtin_nan("");  // negative or NaN
+    }
+    if (x == __builtin_huge_val())
+        return x;
+    int e;
+    double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    const double ln2hi = 0x1.62e42fee00000p-1;  // 21 trailing zero bits: e*ln2hi exact
+    const double ln2lo = 0x1.a39ef35793c76p-33;
+    double f = m - 1.0;
+    double s = f / (2.0 + f);
+    double z = s * s;
+    double R = 0x1.642c8590b2164p-4;
+    R = fma(R, z, 0x1.8618618618618p-4);
+    R = fma(R, z, 0x1.af286bca1af28p-4);
+    R = fma(R, z, 0x1.e1e1e1e1e1e1ep-4);
+    R = fma(R, z, 0x1.1111111111111p-3);
+    R = fma(R, z, 0x1.3b13b13b13b14p-3);
+    R = fma(R, z, 0x1.745d1745d1746p-3);
+    R = fma(R, z, 0x1.c71c71c71c71cp-3);
+    R = fma(R, z, 0x1.2492492492492p-2);
+    R = fma(R, z, 0x1.999999999999ap-2);
+    R = fma(R, z, 0x1.5555555555555p-1);
+    R = R * z;
+    double hfsq = (0.5 * f) * f;
+    double de = (double)e;
+    double lo = fma(s, hfsq + R, de * ln2lo);
+    return de * ln2hi - ((hfsq - lo) - f);
+}
+
+CM_FN double pow(double x, double y)
+{
+    if (y == 0.0 || x == 1.0)
+        return 1.0;
+    if (!(x == x) || !(y == y))
+        return x + y;  // NaN
+    double ax = fabs(x);
+    double ay = fabs(y);
+    bool yint = (y == rint(y));
+    bool yodd = yint && ay < 0x1.0p53 && (0.5 * y != rint(0.5 * y));
+    if (ay == __builtin_huge_val()) {
+        if (ax == 1.0)
+            return 1.0;
+        return ((ax > 1.0) == (y > 0.0)) ? __builtin_huge_val() : 0.0;
+    }
+    if (x < 0.0 && !yint)
+        return __builtin_nan("");
+    if (ax == 0.0) {
+        if (y > 0.0)
+            return yodd ? x : 0.0;
+        return yodd ? __builtin_copysign(__builtin_huge_val(), x) : __builtin_huge_val();
+    }
+    if (ax == __builtin_huge_val()) {
+        double r = (y > 0.0) ? __builtin_huge_val() : 0.0;
+        return (x < 0.0 && yodd) ? -r : r;
+    }
+    double r = cm::exp(y * cm::log(ax));
+    return (x < 0.0 && yodd) ? -r : r;
+}
+
+CM_FN double pown(double x, int n)
+{
+    unsigned int m = n < 0 ? (unsigned int)(-(long long)n) : (unsigned int)n;
+    double r = 1.0;
+    double b = x;
+    while (m) {
+        if (m & 1u)
+            r = r * b;
+        b = b * b;
+        m >>= 1;
+    }
+    return n < 0 ? 1.0 / r : r;
+}
+
+// ----------------------------------------------------------------- float --
+CM_FN float exp(float x)
+{
+    if (!(x == x))
+        return x;
+    if (x > 88.72283935546875f)
+        return __builtin_huge_valf();
+    if (x < -103.97208404541015625f)
+        return 0.0f;
+    const float log2e = 0x1.715476p+0f;
+    const float ln2hi = 0x1.62e430p-1f;
+    const float ln2lo = -0x1.05c610p-29f;
+    float k = rintf(x * log2e);
+    float r = fmaf(-k, ln2hi, x);
+    r = fmaf(-k, ln2lo, r);
+    float p = 0x1.a01a02p-13f;
+    p = fmaf(p, r, 0x1.6c16c2p-10f);
+    p = fmaf(p, r, 0x1.111112p-7f);
+    p = fmaf(p, r, 0x1.555556p-5f);
+    p = fmaf(p, r, 0x1.555556p-3f);
+    p = fmaf(p, r, 0x1.000000p-1f);
+    p = fmaf(p, r, 1.0f);
+    p = fmaf(p, r, 1.0f);
+    return ldexpf(p, (int)k);
+}
+
+CM_FN float log(float x)
+{
+    if (!(x > 0.0f)) {
+        if (x == 0.0f)
+            return -__builtin_huge_valf();
+        return __builtin_nanf("");
+    }
+    if (x == __builtin_huge_valf())
+        return x;
+    int e;
+    float m = frexpf(x, &e);
+    if (m < 0x1.6a09e6p-1f) {
+        m = m * 2.0f;
+        e = e - 1;
+    }
+    const float ln2hi = 0x1.62e400p-1f;  // 15 trailing zero bits: e*ln2hi exact
+    const float ln2lo = 0x1.7f7d1cp-20f;
+    float f = m - 1.0f;
+    float s = f / (2.0f + f);
+    float z = s * s;
+    float R = 0x1.c71c72p-3f;
+    R = fmaf(R, z, 0x1.24924ap-2f);
+    R = fmaf(R, z, 0x1.99999ap-2f);
+    R = fmaf(R, z, 0x1.555556p-1f);
+    R = R * z;
+    float hfsq = (0.5f * f) * f;
+    float de = (float)e;
+    float lo = fmaf(s, hfsq + R, de * ln2lo);
+    return de * ln2hi - ((hfsq - lo) - f);
+}
+
+CM_FN float pow(float x, float y)
+{
+    if (y == 0.0f || x == 1.0f)
+        return 1.0f;
+    if (!(x == x) || !(y == y))
+        return x + y;
+    float ax = fabsf(x);
+    float ay = fabsf(y);
+    bool yint = (y == rintf(y));
+    bool yodd = yint && ay < 0x1.0p24f && (0.5f * y != rintf(0.5f * y));
+    if (ay == __builtin_huge_valf()) {
+        if (ax == 1.0f)
+            return 1.0f;
+        return ((ax > 1.0f) == (y > 0.0f)) ? __builtin_huge_valf() : 0.0f;
+    }
+    if (x < 0.0f && !yint)
+        return __builtin_nanf("");
+    if (ax == 0.0f) {
+        if (y > 0.0f)
+            return yodd ? x : 0.0f;
+        return yodd ? __builtin_copysignf(__builtin_huge_valf(), x) : __builtin_huge_valf();
+    }
+    if (ax == __builtin_huge_valf()) {
+        float r = (y > 0.0f) ? __builtin_huge_valf() : 0.0f;
+        return (x < 0.0f && yodd) ? -r : r;
+    }
+    float r = cm::exp(y * cm::log(ax));
+    return (x < 0.0f && yodd) ? -r : r;
+}
+
+CM_FN float pown(float x, int n)
+{
+    unsigned int m = n < 0 ? (unsigned int)(-(long long)n) : (unsigned int)n;
+    float r = 1.0f;
+    float b = x;
+    while (m) {
+        if (m & 1u)
+            r = r * b;
+        b = b * b;
+        m >>= 1;
+    }
+    return n < 0 ? 1.0f / r : r;
+}
+
+}  // namespace cm
+
+#endif  // CLODE_CM_MATH_H

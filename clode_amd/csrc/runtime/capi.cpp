@@ -1,0 +1,561 @@
+// extern "C" entry points declared in include/clode_hip.h.  Each wraps one
+// Simulator member; exceptions become status codes plus a thread-local
+// message (InvalidArgument -> CLODE_ERR_INVALID, everything else ->
+// CLODE_ERR_RUNTIME).
+#include "../../../include/clode_hip.h"
+
+#include <cstring>
+
+#include "simulator.h"
+
+using namespace clode;
+
+struct clode_sim {
+    Simulator *s;
+};
+
+static thread_local std::string g_last_error;
+
+#define CLODE_TRY try {
+#define CLODE_CATCH                                  \
+    }                                                \
+    catch (const InvalidArgument &e)                 \
+    {                                                \
+        g_last_error = e.what();                     \
+        return CLODE_ERR_INVALID;                    \
+    }                                                \
+    catch (const std::out_of_range &e)               \
+    {                                                \
+        g_last_error = e.what();                     \
+        return CLODE_ERR_INVALID;                    \
+    }                                                \
+    catch (const std::exception &e)                  \
+    {                                                \
+        g_last_error = e.what();                     \
+        return CLODE_ERR_RUNTIME;                    \
+    }                                                \
+    catch (...)                                      \
+    {                                                \
+        g_last_error = "unknown error";              \
+        return CLODE_ERR_RUNTIME;                    \
+    }                                                \
+    return CLODE_OK;
+
+#define CHECK_SIM(sim)                                        \
+    if (!(sim) || !(sim)->s) {                                \
+        g_last_error = "null simulator handle";               \
+        return CLODE_ERR_INVALID;                             \
+    }
+
+static std::vector<std::string> split_names(const char *s)
+{
+    std::vector<std::string> out;
+    if (!s || !*s)
+        return out;
+    std::string cur;
+    for (const char *p = s; *p; ++p) {
+        if (*p == '\n') {
+            out.push_back(cur);
+            cur.clear();
+        } else {
+            cur += *p;
+        }
+    }
+    out.push_back(cur);
+    return out;
+}
+
+static ProblemInfo to_pi(const clode_problem_info *pi)
+{
+    if (!pi || !pi->src_file)
+        throw InvalidArgument("problem info with a source file is required");
+    ProblemInfo p;
+    p.src_file = pi->src_file;
+    p.nVar = pi->num_var;
+    p.nPar = pi->num_par;
+    p.nAux = pi->num_aux;
+    p.nWiener = pi->num_noise;
+    p.varNames = split_names(pi->var_names);
+    p.parNames = split_names(pi->par_names);
+    p.auxNames = split_names(pi->aux_names);
+    return p;
+}
+
+static ObserverParams to_op(const clode_observer_params *o)
+{
+    ObserverParams p;
+    if (!o)
+        return p;
+    p.eVarIx = o->eVarIx;
+    p.fVarIx = o->fVarIx;
+    p.maxEventCount = o->maxEventCount;
+    p.maxEventTimestamps = o->maxEventTimestamps;
+    p.minXamp = o->minXamp;
+    p.minIMI = o->minIMI;
+    p.nHoodRadius = o->nHoodRadius;
+    p.xUpThresh = o->xUpThresh;
+    p.xDownThresh = o->xDownThresh;
+    p.dxUpThresh = o->dxUpThresh;
+    p.dxDownThresh = o->dxDownThresh;
+    p.eps_dx = o->eps_dx;
+    return p;
+}
+
+static int copy_string(const std::string &s, char *buf, size_t cap, size_t *needed)
+{
+    if (needed)
+        *needed = s.size() + 1;
+    if (buf && cap > 0) {
+        size_t n = std::min(cap - 1, s.size());
+        std::memcpy(buf, s.data(), n);
+        buf[n] = '\0';
+    }
+    return CLODE_OK;
+}
+
+static std::string join(const std::vector<std::string> &v)
+{
+    std::string s;
+    for (size_t i = 0; i < v.size(); ++i) {
+        if (i)
+            s += '\n';
+        s += v[i];
+    }
+    return s;
+}
+
+static int copy_vec(const std::vector<double> &v, double *out, size_t n)
+{
+    if (!out && n > 0) {
+        g_last_error = "null output buffer";
+        return CLODE_ERR_INVALID;
+    }
+    if (n < v.size()) {
+        g_last_error = "output buffer too small: need " + std::to_string(v.size());
+        return CLODE_ERR_INVALID;
+    }
+    std::copy(v.begin(), v.end(), out);
+    return CLODE_OK;
+}
+
+extern "C" {
+
+const char *clode_last_error(void) { return g_last_error.c_str(); }
+const char *clode_version(void) { return "clode_amd 0.1.0 (gfx950, hipRTC)"; }
+
+int clode_device_count(int *count)
+{
+    CLODE_TRY
+    *count = device_count();
+    CLODE_CATCH
+}
+
+int clode_device_query(int device, clode_device_info *info)
+{
+    CLODE_TRY
+    if (!info)
+        throw InvalidArgument("null info");
+    int n = device_count();
+    if (device < 0 || device >= n)
+        throw InvalidArgument("device ordinal out of range");
+    hipDeviceProp_t p;
+    hip_check(hipGetDeviceProperties(&p, device), "hipGetDeviceProperties");
+    std::memset(info, 0, sizeof(*info));
+    std::strncpy(info->name, p.name, sizeof(info->name) - 1);
+    std::strncpy(info->arch, p.gcnArchName, sizeof(info->arch) - 1);
+    info->compute_units = p.multiProcessorCount;
+    info->max_clock_khz = p.clockRate;
+    info->max_work_group_size = p.maxThreadsPerBlock;
+    info->device_memory_size = p.totalGlobalMem;
+    info->max_memory_alloc_size = p.totalGlobalMem;
+    info->double_support = 1;
+    CLODE_CATCH
+}
+
+int clode_create(clode_sim **out, int kind, const clode_problem_info *pi, const char *stepper, const char *observer,
+                 const clode_observer_params *op, int precision, int device, const char *clode_root)
+{
+    CLODE_TRY
+    if (!out)
+        throw InvalidArgument("null output handle");
+    *out = nullptr;
+    if (kind < CLODE_KIND_BASE || kind > CLODE_KIND_TRAJECTORY)
+        throw InvalidArgument("unknown simulator kind");
+    Simulator *s = new Simulator(kind, to_pi(pi), stepper ? stepper : "rk4", observer ? observer : "",
+                                 to_op(op), precision, device, clode_root ? clode_root : "");
+    *out = new clode_sim{s};
+    CLODE_CATCH
+}
+
+int clode_destroy(clode_sim *sim)
+{
+    CLODE_TRY
+    if (sim) {
+        delete sim->s;
+        delete sim;
+    }
+    CLODE_CATCH
+}
+
+int clode_set_problem_info(clode_sim *sim, const clode_problem_info *pi)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setProblemInfo(to_pi(pi));
+    CLODE_CATCH
+}
+int clode_set_stepper(clode_sim *sim, const char *stepper)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setStepper(stepper ? stepper : "");
+    CLODE_CATCH
+}
+int clode_set_precision(clode_sim *sim, int precision)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setPrecision(precision);
+    CLODE_CATCH
+}
+int clode_set_device(clode_sim *sim, int device)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setDevice(device);
+    CLODE_CATCH
+}
+int clode_build(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->build();
+    CLODE_CATCH
+}
+int clode_compile(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->compileOnly();
+    CLODE_CATCH
+}
+
+int clode_set_problem_data(clode_sim *sim, const double *x0, size_t nx0, const double *pars, size_t npars)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    std::vector<double> a(x0, x0 + nx0), b(pars, pars + npars);
+    sim->s->setProblemData(a, b);
+    CLODE_CATCH
+}
+int clode_set_tspan(clode_sim *sim, const double *tspan, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setTspan(std::vector<double>(tspan, tspan + n));
+    CLODE_CATCH
+}
+int clode_set_x0(clode_sim *sim, const double *x0, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setX0(std::vector<double>(x0, x0 + n));
+    CLODE_CATCH
+}
+int clode_set_pars(clode_sim *sim, const double *pars, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setPars(std::vector<double>(pars, pars + n));
+    CLODE_CATCH
+}
+int clode_set_solver_params(clode_sim *sim, const clode_solver_params *sp)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (!sp)
+        throw InvalidArgument("null solver params");
+    SolverParams p;
+    p.dt = sp->dt;
+    p.dtmax = sp->dtmax;
+    p.abstol = sp->abstol;
+    p.reltol = sp->reltol;
+    p.max_steps = sp->max_steps;
+    p.max_store = sp->max_store;
+    p.nout = sp->nout;
+    sim->s->setSolverParams(p);
+    CLODE_CATCH
+}
+int clode_seed_rng(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->seedRNG();
+    CLODE_CATCH
+}
+int clode_seed_rng_int(clode_sim *sim, int seed)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->seedRNG(seed);
+    CLODE_CATCH
+}
+
+int clode_transient(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->transient();
+    CLODE_CATCH
+}
+int clode_shift_tspan(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->shiftTspan();
+    CLODE_CATCH
+}
+int clode_shift_x0(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->shiftX0();
+    CLODE_CATCH
+}
+
+int clode_get_n_pts(clode_sim *sim, int *n)
+{
+    CHECK_SIM(sim) CLODE_TRY *n = sim->s->nPts;
+    CLODE_CATCH
+}
+int clode_get_tspan(clode_sim *sim, double *t)
+{
+    CHECK_SIM(sim) CLODE_TRY t[0] = sim->s->tspan[0];
+    t[1] = sim->s->tspan[1];
+    CLODE_CATCH
+}
+int clode_get_solver_params(clode_sim *sim, clode_solver_params *sp)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    const SolverParams &p = sim->s->sp;
+    sp->dt = p.dt;
+    sp->dtmax = p.dtmax;
+    sp->abstol = p.abstol;
+    sp->reltol = p.reltol;
+    sp->max_steps = p.max_steps;
+    sp->max_store = p.max_store;
+    sp->nout = p.nout;
+    CLODE_CATCH
+}
+int clode_get_pars(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->pars, out, n);
+    CLODE_CATCH
+}
+int clode_get_x0(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getX0(), out, n);
+    CLODE_CATCH
+}
+int clode_get_xf(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getXf(), out, n);
+    CLODE_CATCH
+}
+int clode_get_dt(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getDt(), out, n);
+    CLODE_CATCH
+}
+int clode_get_tf(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getTf(), out, n);
+    CLODE_CATCH
+}
+int clode_get_available_steppers(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CLODE_TRY
+    std::vector<std::string> v;
+    for (const auto &kv : stepper_map())
+        v.push_back(kv.first);
+    return copy_string(join(v), buf, cap, needed);
+    CLODE_CATCH
+}
+int clode_get_program_string(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_string(sim->s->programString(), buf, cap, needed);
+    CLODE_CATCH
+}
+int clode_get_problem_info(clode_sim *sim, int *nvar, int *npar, int *naux, int *nnoise)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    *nvar = sim->s->prob.nVar;
+    *npar = sim->s->prob.nPar;
+    *naux = sim->s->prob.nAux;
+    *nnoise = sim->s->prob.nWiener;
+    CLODE_CATCH
+}
+int clode_get_precision(clode_sim *sim, int *precision)
+{
+    CHECK_SIM(sim) CLODE_TRY *precision = sim->s->prec;
+    CLODE_CATCH
+}
+int clode_get_stepper(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_string(sim->s->stepper, buf, cap, needed);
+    CLODE_CATCH
+}
+int clode_print_status(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    Simulator &s = *sim->s;
+    log_msg(LOG_INFO, "------------------");
+    log_msg(LOG_INFO, "   " + s.prob.src_file);
+    log_msg(LOG_INFO, "   nVar=" + std::to_string(s.prob.nVar));
+    log_msg(LOG_INFO, "   nPar=" + std::to_string(s.prob.nPar));
+    log_msg(LOG_INFO, "   nAux=" + std::to_string(s.prob.nAux));
+    log_msg(LOG_INFO, "   nWiener=" + std::to_string(s.prob.nWiener));
+    log_msg(LOG_INFO, std::string("Using ") +
+                          (s.prec == PREC_SINGLE ? "single" : s.prec == PREC_DOUBLE ? "double" : "fp32-compensated") +
+                          " precision.");
+    log_msg(LOG_INFO, "Using stepper: " + s.stepper);
+    CLODE_CATCH
+}
+
+int clode_set_observer_params(clode_sim *sim, const clode_observer_params *op)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (!op)
+        throw InvalidArgument("null observer params");
+    sim->s->setObserverParams(to_op(op));
+    CLODE_CATCH
+}
+int clode_set_observer(clode_sim *sim, const char *observer)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setObserver(observer ? observer : "");
+    CLODE_CATCH
+}
+int clode_initialize_observer(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->initializeObserver();
+    CLODE_CATCH
+}
+int clode_is_observer_initialized(clode_sim *sim, int *initialized)
+{
+    CHECK_SIM(sim) CLODE_TRY *initialized = sim->s->observerInitialized ? 1 : 0;
+    CLODE_CATCH
+}
+int clode_features(clode_sim *sim, int reinitialize)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (sim->s->kind != CLODE_KIND_FEATURES)
+        throw InvalidArgument("features() needs a feature simulator");
+    if (reinitialize < 0)
+        sim->s->features();
+    else
+        sim->s->features(reinitialize != 0);
+    CLODE_CATCH
+}
+int clode_get_observer_params(clode_sim *sim, clode_observer_params *o)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    const ObserverParams &p = sim->s->op;
+    o->eVarIx = p.eVarIx;
+    o->fVarIx = p.fVarIx;
+    o->maxEventCount = p.maxEventCount;
+    o->maxEventTimestamps = p.maxEventTimestamps;
+    o->minXamp = p.minXamp;
+    o->minIMI = p.minIMI;
+    o->nHoodRadius = p.nHoodRadius;
+    o->xUpThresh = p.xUpThresh;
+    o->xDownThresh = p.xDownThresh;
+    o->dxUpThresh = p.dxUpThresh;
+    o->dxDownThresh = p.dxDownThresh;
+    o->eps_dx = p.eps_dx;
+    CLODE_CATCH
+}
+int clode_get_observer_name(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_string(sim->s->observer, buf, cap, needed);
+    CLODE_CATCH
+}
+int clode_get_n_features(clode_sim *sim, int *n)
+{
+    CHECK_SIM(sim) CLODE_TRY *n = sim->s->nFeatures;
+    CLODE_CATCH
+}
+int clode_get_f(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getF(), out, n);
+    CLODE_CATCH
+}
+int clode_get_feature_names(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_string(join(sim->s->featureNames), buf, cap, needed);
+    CLODE_CATCH
+}
+int clode_get_available_observers(clode_sim *sim, char *buf, size_t cap, size_t *needed)
+{
+    CLODE_TRY
+    std::vector<std::string> v;
+    for (const auto &kv : observer_map())
+        v.push_back(kv.first);
+    return copy_string(join(v), buf, cap, needed);
+    CLODE_CATCH
+}
+
+int clode_trajectory(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (sim->s->kind != CLODE_KIND_TRAJECTORY)
+        throw InvalidArgument("trajectory() needs a trajectory simulator");
+    sim->s->trajectory();
+    CLODE_CATCH
+}
+int clode_get_trajectory_sizes(clode_sim *sim, size_t *nt, size_t *nx, size_t *naux)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    *nt = sim->s->telements;
+    *nx = sim->s->xelements;
+    *naux = sim->s->auxelements;
+    CLODE_CATCH
+}
+int clode_get_t(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getT(), out, n);
+    CLODE_CATCH
+}
+int clode_get_x(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getX(), out, n);
+    CLODE_CATCH
+}
+int clode_get_dx(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getDx(), out, n);
+    CLODE_CATCH
+}
+int clode_get_aux(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getAux(), out, n);
+    CLODE_CATCH
+}
+int clode_get_n_stored(clode_sim *sim, int *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    std::vector<int> v = sim->s->getNstored();
+    if (n < v.size())
+        throw InvalidArgument("output buffer too small");
+    std::copy(v.begin(), v.end(), out);
+    CLODE_CATCH
+}
+
+int clode_set_log_level(int level)
+{
+    CLODE_TRY set_log_level(level);
+    CLODE_CATCH
+}
+int clode_get_log_level(int *level)
+{
+    CLODE_TRY *level = get_log_level();
+    CLODE_CATCH
+}
+
+int clode_set_launch(clode_sim *sim, int mode, int block_size)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (mode < 0 || mode > 1)
+        throw InvalidArgument("launch mode must be 0 (grid) or 1 (persistent)");
+    if (block_size < 0 || block_size > 256 || (block_size % 64) != 0)
+        throw InvalidArgument("block size must be 0 (auto) or a multiple of 64 up to 256");
+    sim->s->launchMode = mode;
+    sim->s->blockSize = block_size;
+    CLODE_CATCH
+}
+int clode_set_member_tspan(clode_sim *sim, const double *tspan_m, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setMemberTspan(tspan_m, n);
+    CLODE_CATCH
+}
+int clode_set_dt_refresh(clode_sim *sim, int enable)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->dtRefresh = enable != 0;
+    CLODE_CATCH
+}
+int clode_last_kernel_ms(clode_sim *sim, double *ms)
+{
+    CHECK_SIM(sim) CLODE_TRY *ms = sim->s->lastKernelMs;
+    CLODE_CATCH
+}
+
+}  // extern "C"

@@ -1,0 +1,384 @@
+// Logging, registries, device buffers and the hipRTC program cache.
+//
+// Program assembly follows the reference's scheme (clode/cpp/CLODE.cpp:109-152,
+// CLODEfeatures.cpp:34-39): compile-time defines for precision, stepper,
+// observer and problem sizes, then the user's getRHS source, then the
+// drivers.  The reference builds OpenCL C with clBuildProgram
+// (clode/cpp/OpenCLResource.cpp:260-296); here hipRTC builds a gfx950 code
+// object which is cached by content hash in memory and on disk.
+#include "clode_runtime.h"
+
+#include <hip/hiprtc.h>
+
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <iostream>
+#include <sstream>
+#include <sys/stat.h>
+#include <unistd.h>
+
+namespace clode {
+
+// embedded device headers: kDeviceSources[] = {name, text}
+#include "device_sources.inc"
+
+// ------------------------------------------------------------------ logging
+static int g_log_level = LOG_WARN;
+static std::mutex g_log_mu;
+
+void set_log_level(int level) { g_log_level = level; }
+int get_log_level() { return g_log_level; }
+
+void log_msg(int level, const std::string &msg)
+{
+    if (level < g_log_level || level >= LOG_OFF)
+        return;
+    static const char *names[] = {"trace", "debug", "info", "warning", "error", "critical"};
+    std::lock_guard<std::mutex> lk(g_log_mu);
+    std::fprintf(stderr, "[clode] [%s] %s\n", names[level], msg.c_str());
+}
+
+void hip_check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess) {
+        std::string m = std::string(what) + ": " + hipGetErrorString(e);
+        log_msg(LOG_ERR, m);
+        throw RuntimeError(m);
+    }
+}
+
+// --------------------------------------------------------------- registries
+const std::map<std::string, int> &stepper_map()
+{
+    static const std::map<std::string, int> m = {
+        {"euler", 1}, {"heun", 2}, {"rk4", 3}, {"bs23", 4}, {"dopri5", 5}, {"seuler", 6}};
+    return m;
+}
+
+const std::map<std::string, int> &observer_map()
+{
+    static const std::map<std::string, int> m = {
+        {"thresh2", CLODE_OBS_THRESH2}, {"localmax", CLODE_OBS_LOCALMAX}, {"nhood2", CLODE_OBS_NHOOD2}};
+    return m;
+}
+
+// feature names, in the order the device writes them
+// (observer_threshold_2.clh:15-53, observer_local_maximum.clh:19-45,
+//  observer_neighborhood_2.clh:14-41)
+std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, int nStore)
+{
+    std::vector<std::string> f;
+    auto var = [&](int j) { return j < (int)pi.varNames.size() ? pi.varNames[j] : "x" + std::to_string(j); };
+    auto aux = [&](int j) { return j < (int)pi.auxNames.size() ? pi.auxNames[j] : "aux" + std::to_string(j); };
+    if (obs == CLODE_OBS_THRESH2) {
+        for (const char *s : {"period", "peaks", "upDuration", "downDuration", "duty", "activeDip"})
+            for (const char *p : {"max ", "min ", "mean "})
+                f.push_back(std::string(p) + s);
+        for (int j = 0; j < pi.nVar; ++j) {
+            f.push_back("max " + var(j));
+            f.push_back("min " + var(j));
+            f.push_back("mean " + var(j));
+            f.push_back("max d" + var(j) + "/dt");
+            f.push_back("min d" + var(j) + "/dt");
+        }
+        for (int j = 0; j < pi.nAux; ++j) {
+            f.push_back("max " + aux(j));
+            f.push_back("min " + aux(j));
+            f.push_back("mean " + aux(j));
+        }
+        for (int j = 0; j < nStore; ++j) {
+            f.push_back("up event time " + std::to_string(j));
+            f.push_back("down event time " + std::to_string(j));
+        }
+        for (const char *s : {"event count", "step count", "max dt", "min dt", "mean dt"})
+            f.push_back(s);
+    } else if (obs == CLODE_OBS_LOCALMAX) {
+        for (const char *s : {"IMI", "amplitude"})
+            for (const char *p : {"max ", "min ", "mean "})
+                f.push_back(std::string(p) + s);
+        for (int j = 0; j < pi.nVar; ++j) {
+            f.push_back("max " + var(j));
+            f.push_back("min " + var(j));
+            f.push_back("mean " + var(j));
+            f.push_back("max d" + var(j) + "/dt");
+            f.push_back("min d" + var(j) + "/dt");
+        }
+        for (int j = 0; j < pi.nAux; ++j) {
+            f.push_back("max " + aux(j));
+            f.push_back("min " + aux(j));
+            f.push_back("mean " + aux(j));
+        }
+        for (int j = 0; j < nStore; ++j) {
+            f.push_back("localmax event time " + std::to_string(j));
+            f.push_back("localmax event evar " + std::to_string(j));
+            f.push_back("localmin event time " + std::to_string(j));
+            f.push_back("localmin event evar " + std::to_string(j));
+        }
+        f.push_back("event count");
+        f.push_back("step count");
+    } else if (obs == CLODE_OBS_NHOOD2) {
+        for (const char *s : {"period", "peaks"})
+            for (const char *p : {"max ", "min ", "mean "})
+                f.push_back(std::string(p) + s);
+        for (int j = 0; j < pi.nVar; ++j) {
+            f.push_back("max " + var(j));
+            f.push_back("min " + var(j));
+            f.push_back("mean " + var(j));
+            f.push_back("range " + var(j));
+            f.push_back("nhood center " + var(j));
+            f.push_back("max d" + var(j) + "/dt");
+            f.push_back("min d" + var(j) + "/dt");
+        }
+        for (int j = 0; j < pi.nAux; ++j) {
+            f.push_back("max " + aux(j));
+            f.push_back("min " + aux(j));
+            f.push_back("mean " + aux(j));
+        }
+        for (int j = 0; j < nStore; ++j)
+            f.push_back("nhood event time " + std::to_string(j));
+        for (const char *s : {"event count", "step count", "max dt", "min dt", "mean dt"})
+            f.push_back(s);
+    }
+    return f;
+}
+
+// ------------------------------------------------------------------- device
+int device_count()
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess)
+        return 0;
+    return n;
+}
+
+void set_device(int device)
+{
+    int n = device_count();
+    if (n <= 0)
+        throw RuntimeError("no HIP device is available");
+    if (device < 0 || device >= n)
+        throw InvalidArgument("device ordinal " + std::to_string(device) + " out of range (" +
+                              std::to_string(n) + " devices)");
+    hip_check(hipSetDevice(device), "hipSetDevice");
+}
+
+void DeviceBuffer::allocate(int dev, size_t nbytes)
+{
+    release();
+    if (nbytes == 0)
+        nbytes = 8;
+    set_device(dev);
+    hip_check(hipMalloc(&ptr, nbytes), "hipMalloc");
+    bytes = nbytes;
+    device = dev;
+}
+
+void DeviceBuffer::release()
+{
+    if (ptr) {
+        hipSetDevice(device);
+        hipFree(ptr);
+    }
+    ptr = nullptr;
+    bytes = 0;
+}
+
+// ------------------------------------------------------------------ program
+std::string compose_source(const ProgramConfig &cfg)
+{
+    std::ostringstream s;
+    s << "// clode_amd hipRTC program\n";
+    s << "#define CLODE_PREC " << cfg.prec << "\n";
+    s << "#define CLODE_STEPPER " << cfg.stepper << "\n";
+    if (cfg.observer > 0) {
+        s << "#define CLODE_OBSERVER " << cfg.observer << "\n";
+        s << "#define CLODE_EVAR " << cfg.eVar << "\n";
+        s << "#define CLODE_FVAR " << cfg.fVar << "\n";
+    }
+    s << "#define N_VAR " << cfg.nVar << "\n";
+    s << "#define N_PAR " << cfg.nPar << "\n";
+    s << "#define N_AUX " << cfg.nAux << "\n";
+    s << "#define N_WIENER " << cfg.nWiener << "\n";
+    s << "#define N_STORE_EVENTS " << cfg.nStore << "\n";
+    s << "#include \"clode_prelude.h\"\n";
+    s << "#pragma clang force_cuda_host_device begin\n";
+    s << "#line 1 \"" << (cfg.rhs_name.empty() ? std::string("rhs.cl") : cfg.rhs_name) << "\"\n";
+    s << cfg.rhs << "\n";
+    s << "#pragma clang force_cuda_host_device end\n";
+    s << "#include \"clode_rng.h\"\n";
+    s << "#include \"clode_steppers.h\"\n";
+    if (cfg.observer > 0)
+        s << "#include \"clode_observers.h\"\n";
+    s << "#include \"clode_kernels.h\"\n";
+    if (cfg.persistent)
+        s << "#include \"clode_persistent.h\"\n";
+    return s.str();
+}
+
+std::vector<std::string> compile_options(const ProgramConfig &cfg)
+{
+    std::vector<std::string> o = {"--offload-arch=gfx950", "-O3", "-std=c++17", "-ffp-contract=on",
+                                  // packed FP32 (v_pk_*_f32) is disabled for all device code
+                                  "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+                                  "-Wno-unused-parameter", "-Wno-unused-variable"};
+    if (cfg.prec == PREC_FP32C)
+        o.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
+    else
+        o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
+    return o;
+}
+
+static uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull)
+{
+    for (unsigned char c : s) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    return h;
+}
+
+static std::mutex g_prog_mu;
+// never destroyed: modules stay loaded until the process exits, so no HIP call
+// runs during static destruction
+static std::map<uint64_t, std::shared_ptr<Program>> &g_programs = *new std::map<uint64_t, std::shared_ptr<Program>>();
+
+static std::string cache_path(const std::string &dir, uint64_t key)
+{
+    char name[64];
+    std::snprintf(name, sizeof(name), "clode_%016llx.co", (unsigned long long)key);
+    std::string d = dir;
+    if (!d.empty() && d.back() != '/')
+        d += '/';
+    return d + name;
+}
+
+static bool read_cache(const std::string &path, uint64_t key, std::vector<char> &code)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        return false;
+    char magic[8];
+    uint64_t k = 0, n = 0;
+    f.read(magic, 8);
+    f.read((char *)&k, 8);
+    f.read((char *)&n, 8);
+    if (!f || std::memcmp(magic, "CLODECO1", 8) != 0 || k != key || n == 0 || n > (1ull << 30))
+        return false;
+    code.resize(n);
+    f.read(code.data(), (std::streamsize)n);
+    return (bool)f;
+}
+
+static void write_cache(const std::string &dir, const std::string &path, uint64_t key, const std::vector<char> &code)
+{
+    if (dir.empty())
+        return;
+    mkdir(dir.c_str(), 0755);
+    std::string tmp = path + ".tmp" + std::to_string(getpid());
+    {
+        std::ofstream f(tmp, std::ios::binary);
+        if (!f)
+            return;
+        uint64_t n = code.size();
+        f.write("CLODECO1", 8);
+        f.write((const char *)&key, 8);
+        f.write((const char *)&n, 8);
+        f.write(code.data(), (std::streamsize)n);
+        if (!f)
+            return;
+    }
+    std::rename(tmp.c_str(), path.c_str());
+}
+
+static void rtc_check(hiprtcResult r, const char *what)
+{
+    if (r != HIPRTC_SUCCESS)
+        throw RuntimeError(std::string(what) + ": " + hiprtcGetErrorString(r));
+}
+
+std::shared_ptr<Program> get_program(const ProgramConfig &cfg, const std::string &cache_dir)
+{
+    auto prog = std::make_shared<Program>();
+    prog->source = compose_source(cfg);
+    prog->options = compile_options(cfg);
+    std::string keytext = prog->source;
+    for (const auto &o : prog->options)
+        keytext += "\n" + o;
+    for (const auto &h : kDeviceSources)
+        keytext += h.text;
+    const uint64_t key = fnv1a(keytext);
+
+    std::lock_guard<std::mutex> lk(g_prog_mu);
+    auto it = g_programs.find(key);
+    if (it != g_programs.end())
+        return it->second;
+
+    const std::string path = cache_dir.empty() ? std::string() : cache_path(cache_dir, key);
+    if (!path.empty() && read_cache(path, key, prog->code)) {
+        log_msg(LOG_DEBUG, "loaded cached code object " + path);
+    } else {
+        std::vector<const char *> hdrs, names;
+        for (const auto &h : kDeviceSources) {
+            hdrs.push_back(h.text);
+            names.push_back(h.name);
+        }
+        hiprtcProgram p;
+        rtc_check(hiprtcCreateProgram(&p, prog->source.c_str(), "clode_program.hip", (int)hdrs.size(),
+                                      hdrs.data(), names.data()),
+                  "hiprtcCreateProgram");
+        std::vector<const char *> opts;
+        for (const auto &o : prog->options)
+            opts.push_back(o.c_str());
+        hiprtcResult r = hiprtcCompileProgram(p, (int)opts.size(), opts.data());
+        size_t logsize = 0;
+        hiprtcGetProgramLogSize(p, &logsize);
+        std::string buildlog(logsize, '\0');
+        if (logsize > 1)
+            hiprtcGetProgramLog(p, &buildlog[0]);
+        if (r != HIPRTC_SUCCESS) {
+            hiprtcDestroyProgram(&p);
+            log_msg(LOG_ERR, "hipRTC build failed:\n" + buildlog);
+            throw RuntimeError("hipRTC build failed: " + buildlog);
+        }
+        if (logsize > 1)
+            log_msg(LOG_DEBUG, "hipRTC build log:\n" + buildlog);
+        size_t n = 0;
+        rtc_check(hiprtcGetCodeSize(p, &n), "hiprtcGetCodeSize");
+        prog->code.resize(n);
+        rtc_check(hiprtcGetCode(p, prog->code.data()), "hiprtcGetCode");
+        hiprtcDestroyProgram(&p);
+        if (!path.empty())
+            write_cache(cache_dir, path, key, prog->code);
+        log_msg(LOG_INFO, "compiled program (" + std::to_string(n) + " bytes)");
+    }
+    g_programs[key] = prog;
+    return prog;
+}
+
+hipFunction_t Program::function(int device, const char *name)
+{
+    std::lock_guard<std::mutex> lk(mu_);
+    set_device(device);
+    hipModule_t mod;
+    auto it = modules_.find(device);
+    if (it == modules_.end()) {
+        hip_check(hipModuleLoadData(&mod, code.data()), "hipModuleLoadData");
+        modules_[device] = mod;
+    } else {
+        mod = it->second;
+    }
+    hipFunction_t fn;
+    hip_check(hipModuleGetFunction(&fn, mod, name), (std::string("hipModuleGetFunction ") + name).c_str());
+    return fn;
+}
+
+Program::~Program()
+{
+    for (auto &kv : modules_) {
+        hipSetDevice(kv.first);
+        hipModuleUnload(kv.second);
+    }
+}
+
+}  // namespace clode

@@ -1,0 +1,631 @@
+// Host semantics of the reference's CLODE / CLODEfeatures / CLODEtrajectory
+// classes on HIP.  Citations point at the reference member each method
+// reproduces.
+#include "simulator.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <fstream>
+#include <random>
+#include <sstream>
+
+namespace clode {
+
+static std::string read_file(const std::string &path)
+{
+    std::ifstream f(path, std::ios::binary);
+    if (!f)
+        throw InvalidArgument("cannot read right-hand-side source file '" + path + "'");
+    std::ostringstream s;
+    s << f.rdbuf();
+    return s.str();
+}
+
+// CLODE::CLODE (CLODE.cpp:13-37), CLODEfeatures::CLODEfeatures (CLODEfeatures.cpp:9-29),
+// CLODEtrajectory::CLODEtrajectory (CLODEtrajectory.cpp:6-20)
+Simulator::Simulator(int kind_, const ProblemInfo &pi, const std::string &stepper_, const std::string &observer_,
+                     const ObserverParams &op_, int precision, int device_, const std::string &root_)
+    : kind(kind_), prec(PREC_DOUBLE), device(device_), root(root_)
+{
+    if (precision < PREC_SINGLE || precision > PREC_FP32C)
+        throw InvalidArgument("precision must be 1 (single), 2 (double) or 3 (fp32c)");
+    prec = precision;
+    setProblemInfo(pi);
+    stepper = "rk4";
+    setStepper(stepper_);
+    if (stepper_map().find(stepper_) == stepper_map().end())
+        throw InvalidArgument("unknown stepper: " + stepper_);
+    if (kind == 1) {
+        if (observer_map().find(observer_) == observer_map().end())
+            throw InvalidArgument("unknown observer: " + observer_);
+        observer = observer_;
+        setObserverParams(op_);
+    }
+    log_msg(LOG_DEBUG, "constructed simulator");
+}
+
+Simulator::~Simulator()
+{
+    if (stream) {
+        hipSetDevice(device);
+        hipStreamSynchronize(stream);
+        hipEventDestroy(ev0);
+        hipEventDestroy(ev1);
+        hipStreamDestroy(stream);
+    }
+}
+
+// CLODE::setProblemInfo (CLODE.cpp:43-59)
+void Simulator::setProblemInfo(const ProblemInfo &pi)
+{
+    if (pi.nVar <= 0)
+        throw InvalidArgument("the problem must have at least one state variable");
+    if (pi.nPar < 0 || pi.nAux < 0 || pi.nWiener < 0)
+        throw InvalidArgument("negative problem dimension");
+    rhsSource = read_file(pi.src_file);
+    prob = pi;
+    if (kind == 1 && !observer.empty())
+        updateObserverInfo();
+}
+
+// CLODE::setStepper (CLODE.cpp:61-76): unknown names leave the stepper unchanged
+void Simulator::setStepper(const std::string &s)
+{
+    if (stepper_map().find(s) != stepper_map().end())
+        stepper = s;
+    else
+        log_msg(LOG_WARN, "Unknown stepper: " + s + ". Stepper method unchanged");
+}
+
+// CLODE::setPrecision (CLODE.cpp:78-86).  Device data of the old precision are
+// released; the caller sets problem data again, as with the reference.
+void Simulator::setPrecision(int p)
+{
+    if (p < PREC_SINGLE || p > PREC_FP32C)
+        throw InvalidArgument("precision must be 1 (single), 2 (double) or 3 (fp32c)");
+    if (p != prec) {
+        prec = p;
+        nPts = 0;
+        Felements = 0;
+        nStoreMax = 0;
+        telements = 0;
+        observerInitialized = false;
+    }
+}
+
+// CLODE::setOpenCL (CLODE.cpp:88-102)
+void Simulator::setDevice(int d)
+{
+    if (d != device) {
+        if (stream) {
+            hipSetDevice(device);
+            hipStreamSynchronize(stream);
+            hipEventDestroy(ev0);
+            hipEventDestroy(ev1);
+            hipStreamDestroy(stream);
+            stream = nullptr;
+        }
+        device = d;
+        nPts = 0;
+        Felements = 0;
+        nStoreMax = 0;
+        telements = 0;
+        observerInitialized = false;
+    }
+}
+
+int Simulator::observerId() const
+{
+    if (kind != 1)
+        return 0;
+    return observer_map().at(observer);
+}
+
+ProgramConfig Simulator::programConfig() const
+{
+    ProgramConfig c;
+    c.prec = prec;
+    c.stepper = stepper_map().at(stepper);
+    c.observer = observerId();
+    c.eVar = kind == 1 ? (int)op.eVarIx : 0;
+    c.fVar = kind == 1 ? (int)op.fVarIx : 0;
+    c.nVar = prob.nVar;
+    c.nPar = prob.nPar;
+    c.nAux = prob.nAux;
+    c.nWiener = prob.nWiener;
+    c.nStore = kind == 1 ? (int)op.maxEventTimestamps : 0;
+    c.persistent = 0;
+    c.rhs = rhsSource;
+    c.rhs_name = prob.src_file;
+    return c;
+}
+
+std::shared_ptr<Program> Simulator::currentProgram()
+{
+    ProgramConfig c = programConfig();
+    std::string key = compose_source(c);
+    if (!program || key != programKeyCache) {
+        program = get_program(c, root.empty() ? std::string() : root);
+        programKeyCache = key;
+    }
+    return program;
+}
+
+// CLODE::buildCL (CLODE.cpp:155-171) + the subclasses' kernel creation
+void Simulator::build()
+{
+    auto p = currentProgram();
+    p->function(device, "clode_transient");
+    if (kind == 1) {
+        p->function(device, "clode_initialize_observer");
+        p->function(device, "clode_features");
+    } else if (kind == 2) {
+        p->function(device, "clode_trajectory");
+    }
+    log_msg(LOG_INFO, "built program for device " + std::to_string(device));
+}
+
+void Simulator::compileOnly() { currentProgram(); }
+
+std::string Simulator::programString()
+{
+    ProgramConfig c = programConfig();
+    std::string s = "// options:";
+    for (const auto &o : compile_options(c))
+        s += " " + o;
+    return s + "\n" + compose_source(c);
+}
+
+void Simulator::ensureStream()
+{
+    set_device(device);
+    if (!stream) {
+        hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+        hip_check(hipEventCreate(&ev0), "hipEventCreate");
+        hip_check(hipEventCreate(&ev1), "hipEventCreate");
+    }
+}
+
+void Simulator::upload(DeviceBuffer &b, const std::vector<double> &v, size_t elem)
+{
+    ensureStream();
+    if (v.empty())
+        return;
+    if (elem == 8) {
+        hip_check(hipMemcpyAsync(b.ptr, v.data(), v.size() * 8, hipMemcpyHostToDevice, stream), "hipMemcpy H2D");
+    } else {
+        std::vector<float> f(v.begin(), v.end());
+        hip_check(hipMemcpyAsync(b.ptr, f.data(), f.size() * 4, hipMemcpyHostToDevice, stream), "hipMemcpy H2D");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        return;
+    }
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+std::vector<double> Simulator::download(const DeviceBuffer &b, size_t n, size_t elem)
+{
+    ensureStream();
+    std::vector<double> out(n);
+    if (n == 0 || !b.ptr)
+        return out;
+    if (elem == 8) {
+        hip_check(hipMemcpyAsync(out.data(), b.ptr, n * 8, hipMemcpyDeviceToHost, stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    } else {
+        std::vector<float> f(n);
+        hip_check(hipMemcpyAsync(f.data(), b.ptr, n * 4, hipMemcpyDeviceToHost, stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::copy(f.begin(), f.end(), out.begin());
+    }
+    return out;
+}
+
+// CLODE::setNpts (CLODE.cpp:174-242)
+void Simulator::setNpts(int newNpts)
+{
+    if (newNpts <= 0)
+        throw InvalidArgument("the ensemble must have at least one member");
+    if (newNpts == nPts)
+        return;
+    set_device(device);
+    size_t total = 0, freeb = 0;
+    hipMemGetInfo(&freeb, &total);
+    size_t largest = (size_t)std::max(prob.nVar, std::max(prob.nPar, prob.nAux)) * (size_t)newNpts * wideSize();
+    if (total && largest > total)
+        throw InvalidArgument("nPts*nVar, nPts*nPar, or nPts*nAux is too large");
+    nPts = newNpts;
+    x0.assign((size_t)prob.nVar * nPts, 0.0);
+    pars.assign((size_t)prob.nPar * nPts, 0.0);
+    rngHost.assign(2 * (size_t)nPts, 0);
+    dtHost.assign(nPts, sp.dt);
+    d_x0.allocate(device, x0.size() * wideSize());
+    d_pars.allocate(device, pars.size() * realSize());
+    d_rng.allocate(device, rngHost.size() * 8);
+    d_dt.allocate(device, (size_t)nPts * wideSize());
+    d_tf.allocate(device, (size_t)nPts * wideSize());
+    d_xf.allocate(device, x0.size() * wideSize());
+    upload(d_dt, dtHost, wideSize());
+    memberTspan = false;
+    d_tspan_m.release();
+    seedRNG();
+    log_msg(LOG_DEBUG, "set nPts=" + std::to_string(nPts));
+}
+
+// CLODE::setProblemData (CLODE.cpp:245-280)
+void Simulator::setProblemData(const std::vector<double> &nx0, const std::vector<double> &npars)
+{
+    if (nx0.size() % prob.nVar != 0) {
+        log_msg(LOG_INFO, "Invalid initial condition vector: not a multiple of nVar=" + std::to_string(prob.nVar));
+        log_msg(LOG_INFO, "...Initial conditions were not updated!");
+        return;
+    }
+    if (prob.nPar > 0 && npars.size() % prob.nPar != 0) {
+        log_msg(LOG_INFO, "Invalid parameter vector: not a multiple of nPar=" + std::to_string(prob.nPar));
+        log_msg(LOG_INFO, "...Parameters were not updated!");
+        return;
+    }
+    int nx = (int)(nx0.size() / prob.nVar);
+    int np = prob.nPar > 0 ? (int)(npars.size() / prob.nPar) : nx;
+    if (nx != np) {
+        log_msg(LOG_INFO, "Initial contition and parameter vector dimensions don't match");
+        log_msg(LOG_INFO, "...Problem data was not updated!");
+        return;
+    }
+    setNpts(nx);
+    setX0(nx0);
+    setPars(npars);
+}
+
+// CLODE::setX0 (CLODE.cpp:283-315)
+void Simulator::setX0(const std::vector<double> &nx0)
+{
+    if (nPts > 0 && nx0.size() == (size_t)nPts * prob.nVar) {
+        x0 = nx0;
+        upload(d_x0, x0, wideSize());
+    } else {
+        log_msg(LOG_INFO, "...Initial conditions were not updated!");
+    }
+}
+
+// CLODE::setPars (CLODE.cpp:318-350)
+void Simulator::setPars(const std::vector<double> &np)
+{
+    if (nPts > 0 && np.size() == (size_t)nPts * prob.nPar) {
+        pars = np;
+        upload(d_pars, pars, realSize());
+    } else {
+        log_msg(LOG_INFO, "Invalid parameter vector: Expected " + std::to_string(nPts) + "*" +
+                              std::to_string(prob.nPar) + " elements, received " + std::to_string(np.size()));
+        log_msg(LOG_INFO, "...Parameters were not updated!");
+    }
+}
+
+// CLODE::setTspan (CLODE.cpp:352-375)
+void Simulator::setTspan(const std::vector<double> &t)
+{
+    if (t.size() != 2)
+        throw InvalidArgument("tspan must have two elements");
+    tspan = t;
+}
+
+// CLODE::setSolverParams (CLODE.cpp:377-403): the host dt vector is refilled
+// but the device buffer is not (unless dt refresh is enabled)
+void Simulator::setSolverParams(const SolverParams &nsp)
+{
+    if (nsp.nout == 0)
+        throw InvalidArgument("nout must be positive");
+    sp = nsp;
+    std::fill(dtHost.begin(), dtHost.end(), sp.dt);
+    if (dtRefresh && nPts > 0)
+        upload(d_dt, dtHost, wideSize());
+}
+
+// CLODE::seedRNG() (CLODE.cpp:420-444)
+void Simulator::seedRNG()
+{
+    if (nPts <= 0)
+        return;
+    std::random_device rd;
+    std::mt19937_64 gen(rd());
+    std::uniform_int_distribution<unsigned long long> dis;
+    for (auto &w : rngHost)
+        w = dis(gen);
+    ensureStream();
+    hip_check(hipMemcpyAsync(d_rng.ptr, rngHost.data(), rngHost.size() * 8, hipMemcpyHostToDevice, stream),
+              "hipMemcpy H2D");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+// CLODE::seedRNG(int) (CLODE.cpp:447-464): word k = seed + k in int arithmetic
+void Simulator::seedRNG(int seed)
+{
+    if (nPts <= 0)
+        return;
+    for (size_t k = 0; k < rngHost.size(); ++k) {
+        int w = (int)((unsigned int)seed + (unsigned int)k);
+        rngHost[k] = (unsigned long long)(long long)w;
+    }
+    ensureStream();
+    hip_check(hipMemcpyAsync(d_rng.ptr, rngHost.data(), rngHost.size() * 8, hipMemcpyHostToDevice, stream),
+              "hipMemcpy H2D");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+void Simulator::setMemberTspan(const double *t, size_t n)
+{
+    if (!t) {
+        memberTspan = false;
+        d_tspan_m.release();
+        return;
+    }
+    requireData("set_member_tspan");
+    if (n != 2 * (size_t)nPts)
+        throw InvalidArgument("member tspan must have 2*nPts elements");
+    d_tspan_m.allocate(device, n * 8);
+    ensureStream();
+    hip_check(hipMemcpyAsync(d_tspan_m.ptr, t, n * 8, hipMemcpyHostToDevice, stream), "hipMemcpy H2D");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    memberTspan = true;
+}
+
+void Simulator::requireData(const char *what) const
+{
+    if (nPts <= 0)
+        throw InvalidArgument(std::string(what) + ": problem data (x0, pars) must be set first");
+}
+
+ClodeArgs Simulator::makeArgs()
+{
+    ClodeArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.t0 = tspan[0];
+    a.t1 = tspan[1];
+    a.sp_dt = sp.dt;
+    a.sp_dtmax = sp.dtmax;
+    a.sp_abstol = sp.abstol;
+    a.sp_reltol = sp.reltol;
+    a.sp_max_steps = sp.max_steps;
+    a.sp_max_store = sp.max_store;
+    a.sp_nout = sp.nout;
+    a.nPts = (unsigned int)nPts;
+    a.op_eVarIx = op.eVarIx;
+    a.op_fVarIx = op.fVarIx;
+    a.op_maxEventCount = op.maxEventCount;
+    a.op_maxEventTimestamps = op.maxEventTimestamps;
+    a.op_minXamp = op.minXamp;
+    a.op_minIMI = op.minIMI;
+    a.op_nHoodRadius = op.nHoodRadius;
+    a.op_xUpThresh = op.xUpThresh;
+    a.op_xDownThresh = op.xDownThresh;
+    a.op_dxUpThresh = op.dxUpThresh;
+    a.op_dxDownThresh = op.dxDownThresh;
+    a.op_eps_dx = op.eps_dx;
+    a.x0 = d_x0.ptr;
+    a.pars = d_pars.ptr;
+    a.xf = d_xf.ptr;
+    a.rng = d_rng.as<unsigned long long>();
+    a.dt = d_dt.ptr;
+    a.tf = d_tf.ptr;
+    a.odr = d_odr.ptr;
+    a.odu = d_odu.as<unsigned int>();
+    a.F = d_F.ptr;
+    a.tr_t = d_t.ptr;
+    a.tr_x = d_x.ptr;
+    a.tr_dx = d_dx.ptr;
+    a.tr_aux = d_aux.ptr;
+    a.nStored = d_nStored.as<int>();
+    a.tspan_m = memberTspan ? d_tspan_m.ptr : nullptr;
+    a.member_offset = 0;
+    a.n_launch = (unsigned int)nPts;
+    return a;
+}
+
+void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads)
+{
+    auto p = currentProgram();
+    hipFunction_t fn = p->function(device, kernel);
+    ensureStream();
+    unsigned int block = blockSize > 0 ? (unsigned int)blockSize : 64u;
+    unsigned int grid = (nThreads + block - 1) / block;
+    ClodeArgs args = a;
+    void *params[] = {&args};
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr),
+              (std::string("launch ") + kernel).c_str());
+    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    hip_check(hipStreamSynchronize(stream), (std::string("kernel ") + kernel).c_str());
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    lastKernelMs = ms;
+}
+
+// CLODE::transient (CLODE.cpp:468-493)
+void Simulator::transient()
+{
+    requireData("transient");
+    launch("clode_transient", makeArgs(), (unsigned int)nPts);
+    log_msg(LOG_INFO, "run transient");
+}
+
+// CLODE::shiftTspan (CLODE.cpp:495-500)
+void Simulator::shiftTspan()
+{
+    std::vector<double> t = {tspan[1], tspan[1] + (tspan[1] - tspan[0])};
+    setTspan(t);
+}
+
+// CLODE::shiftX0 (CLODE.cpp:502-515): device-to-device copy xf -> x0
+void Simulator::shiftX0()
+{
+    requireData("shift_x0");
+    ensureStream();
+    hip_check(hipMemcpyAsync(d_x0.ptr, d_xf.ptr, (size_t)prob.nVar * nPts * wideSize(), hipMemcpyDeviceToDevice,
+                             stream),
+              "hipMemcpy D2D");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+// CLODE::getX0 / getXf / getDt / getTf (CLODE.cpp:517-583)
+std::vector<double> Simulator::getX0()
+{
+    if (nPts <= 0)
+        return {};
+    x0 = download(d_x0, (size_t)prob.nVar * nPts, wideSize());
+    return x0;
+}
+std::vector<double> Simulator::getXf()
+{
+    if (nPts <= 0)
+        return {};
+    return download(d_xf, (size_t)prob.nVar * nPts, wideSize());
+}
+std::vector<double> Simulator::getDt()
+{
+    if (nPts <= 0)
+        return {};
+    return download(d_dt, (size_t)nPts, wideSize());
+}
+std::vector<double> Simulator::getTf()
+{
+    if (nPts <= 0)
+        return {};
+    return download(d_tf, (size_t)nPts, wideSize());
+}
+
+// ------------------------------------------------------------------ features
+
+// CLODEfeatures::updateObserverDefineMap (CLODEfeatures.cpp:108-122)
+void Simulator::updateObserverInfo()
+{
+    int obs = observerId();
+    int S = (int)op.maxEventTimestamps;
+    nFeatures = clode_obs_nfeatures(obs, prob.nVar, prob.nAux, S);
+    featureNames = observer_feature_names(obs, prob, S);
+}
+
+// CLODEfeatures::setObserverParams (CLODEfeatures.cpp:78-106)
+void Simulator::setObserverParams(const ObserverParams &nop)
+{
+    if (nop.eVarIx >= (unsigned int)prob.nVar || nop.fVarIx >= (unsigned int)prob.nVar)
+        throw InvalidArgument("observer eVarIx/fVarIx must index a state variable");
+    op = nop;
+    updateObserverInfo();
+}
+
+// CLODEfeatures::setObserver (CLODEfeatures.cpp:63-76)
+void Simulator::setObserver(const std::string &name)
+{
+    if (observer_map().find(name) != observer_map().end()) {
+        observer = name;
+        updateObserverInfo();
+    } else {
+        log_msg(LOG_WARN, "unknown observer: " + name + ". Observer method unchanged");
+    }
+}
+
+// CLODEfeatures::resizeFeaturesVariables (CLODEfeatures.cpp:143-177)
+void Simulator::resizeFeaturesVariables()
+{
+    int obs = observerId();
+    int S = (int)op.maxEventTimestamps;
+    size_t nr = (size_t)clode_obs_nreal(obs, prob.nVar, prob.nAux, S);
+    size_t nu = (size_t)clode_obs_nuint(obs);
+    size_t fe = (size_t)nFeatures * nPts;
+    if (fe != Felements || nr != odrFields || nu != oduFields || !d_F.ptr) {
+        Felements = fe;
+        odrFields = nr;
+        oduFields = nu;
+        d_odr.allocate(device, nr * nPts * wideSize());
+        d_odu.allocate(device, nu * nPts * 4);
+        d_F.allocate(device, fe * wideSize());
+        observerInitialized = false;
+    }
+}
+
+// CLODEfeatures::initializeObserver (CLODEfeatures.cpp:180-212)
+void Simulator::initializeObserver()
+{
+    requireData("initialize_observer");
+    resizeFeaturesVariables();
+    launch("clode_initialize_observer", makeArgs(), (unsigned int)nPts);
+    observerInitialized = true;
+}
+
+// CLODEfeatures::features(bool) (CLODEfeatures.cpp:215-220)
+void Simulator::features(bool reinit)
+{
+    observerInitialized = !reinit;
+    features();
+}
+
+// CLODEfeatures::features() (CLODEfeatures.cpp:222-259)
+void Simulator::features()
+{
+    requireData("features");
+    resizeFeaturesVariables();
+    if (!observerInitialized)
+        initializeObserver();
+    launch("clode_features", makeArgs(), (unsigned int)nPts);
+}
+
+// CLODEfeatures::getF (CLODEfeatures.cpp:261-275)
+std::vector<double> Simulator::getF()
+{
+    if (nPts <= 0 || !d_F.ptr)
+        return std::vector<double>(Felements, 0.0);
+    return download(d_F, Felements, wideSize());
+}
+
+// ---------------------------------------------------------------- trajectory
+
+// CLODEtrajectory::resizeTrajectoryVariables (CLODEtrajectory.cpp:45-94)
+void Simulator::resizeTrajectoryVariables()
+{
+    unsigned int store = sp.max_store;
+    size_t largest = (size_t)std::max(1, std::max(prob.nVar, prob.nAux)) * nPts * store * realSize();
+    size_t freeb = 0, total = 0;
+    set_device(device);
+    hipMemGetInfo(&freeb, &total);
+    if (total && largest > total)
+        throw InvalidArgument("nPts*nStoreMax*nVar*realSize or nPts*nStoreMax*nAux*realSize is too big");
+    size_t te = (size_t)store * nPts;
+    if (nStoreMax != store || telements != te || !d_t.ptr) {
+        nStoreMax = store;
+        telements = te;
+        xelements = (size_t)prob.nVar * te;
+        auxelements = prob.nAux > 0 ? (size_t)prob.nAux * te : 1;
+        d_t.allocate(device, te * wideSize());
+        d_x.allocate(device, xelements * realSize());
+        d_dx.allocate(device, xelements * realSize());
+        d_aux.allocate(device, auxelements * realSize());
+        d_nStored.allocate(device, (size_t)nPts * 4);
+    }
+}
+
+// CLODEtrajectory::trajectory (CLODEtrajectory.cpp:97-130)
+void Simulator::trajectory()
+{
+    requireData("trajectory");
+    resizeTrajectoryVariables();
+    launch("clode_trajectory", makeArgs(), (unsigned int)nPts);
+}
+
+// CLODEtrajectory::getT / getX / getDx / getAux / getNstored (CLODEtrajectory.cpp:132-205)
+std::vector<double> Simulator::getT() { return download(d_t, telements, wideSize()); }
+std::vector<double> Simulator::getX() { return download(d_x, xelements, realSize()); }
+std::vector<double> Simulator::getDx() { return download(d_dx, xelements, realSize()); }
+std::vector<double> Simulator::getAux() { return download(d_aux, auxelements, realSize()); }
+std::vector<int> Simulator::getNstored()
+{
+    std::vector<int> out(nPts > 0 ? nPts : 0);
+    if (!d_nStored.ptr || out.empty())
+        return out;
+    ensureStream();
+    hip_check(hipMemcpyAsync(out.data(), d_nStored.ptr, out.size() * 4, hipMemcpyDeviceToHost, stream),
+              "hipMemcpy D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return out;
+}
+
+}  // namespace clode

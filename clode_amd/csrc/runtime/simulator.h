@@ -1,0 +1,127 @@
+// Simulator state and operations behind the C ABI.
+//
+// One class covers the three reference host classes (kind selects the
+// extra members): CLODE (clode/cpp/CLODE.hpp:93-189), CLODEfeatures
+// (clode/cpp/CLODEfeatures.hpp) and CLODEtrajectory
+// (clode/cpp/CLODEtrajectory.hpp).  Host-side semantics are the reference's:
+// size-mismatched inputs are ignored with an info message, the device step
+// buffer is refilled only when the ensemble size changes, the RNG is reseeded
+// (non-deterministically) whenever the ensemble size changes, observer data
+// survive between features() calls until the ensemble size changes or a
+// re-initialisation is requested, and every compute call blocks.
+#pragma once
+
+#include "clode_runtime.h"
+
+namespace clode {
+
+class Simulator {
+public:
+    Simulator(int kind, const ProblemInfo &pi, const std::string &stepper, const std::string &observer,
+              const ObserverParams &op, int precision, int device, const std::string &root);
+    ~Simulator();
+
+    // configuration
+    void setProblemInfo(const ProblemInfo &pi);
+    void setStepper(const std::string &s);
+    void setPrecision(int prec);
+    void setDevice(int device);
+    void build();
+    void compileOnly();
+
+    // data
+    void setProblemData(const std::vector<double> &x0, const std::vector<double> &pars);
+    void setTspan(const std::vector<double> &t);
+    void setX0(const std::vector<double> &x0);
+    void setPars(const std::vector<double> &pars);
+    void setSolverParams(const SolverParams &sp);
+    void seedRNG();
+    void seedRNG(int seed);
+    void setMemberTspan(const double *tspan_m, size_t n);
+
+    // compute
+    void transient();
+    void shiftTspan();
+    void shiftX0();
+
+    // getters
+    std::vector<double> getX0();
+    std::vector<double> getXf();
+    std::vector<double> getDt();
+    std::vector<double> getTf();
+    std::string programString();
+
+    // features
+    void setObserverParams(const ObserverParams &op);
+    void setObserver(const std::string &name);
+    void initializeObserver();
+    void features();
+    void features(bool reinit);
+    std::vector<double> getF();
+
+    // trajectory
+    void trajectory();
+    std::vector<double> getT();
+    std::vector<double> getX();
+    std::vector<double> getDx();
+    std::vector<double> getAux();
+    std::vector<int> getNstored();
+
+    // public state (read by the C ABI)
+    int kind;
+    ProblemInfo prob;
+    std::string rhsSource;
+    std::string stepper;
+    int prec;
+    int device;
+    std::string root;
+    SolverParams sp;
+    std::vector<double> tspan{0.0, 1000.0};
+    int nPts = 0;
+    std::vector<double> x0, pars, dtHost;
+    std::vector<unsigned long long> rngHost;
+
+    std::string observer;
+    ObserverParams op;
+    bool observerInitialized = false;
+    int nFeatures = 0;
+    std::vector<std::string> featureNames;
+
+    unsigned int nStoreMax = 0;
+    size_t telements = 0, xelements = 0, auxelements = 0;
+
+    int launchMode = 0;   // 0 grid, 1 persistent work queue
+    int blockSize = 0;    // 0 = auto
+    bool dtRefresh = false;
+    double lastKernelMs = 0.0;
+
+private:
+    DeviceBuffer d_x0, d_pars, d_xf, d_rng, d_dt, d_tf;
+    DeviceBuffer d_odr, d_odu, d_F;
+    DeviceBuffer d_t, d_x, d_dx, d_aux, d_nStored;
+    DeviceBuffer d_work, d_tspan_m;
+    size_t odrFields = 0, oduFields = 0, Felements = 0;
+    bool memberTspan = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    std::shared_ptr<Program> program;
+    std::string programKeyCache;
+
+    size_t realSize() const { return prec == PREC_DOUBLE ? 8 : 4; }   // pars, trajectory x
+    size_t wideSize() const { return prec == PREC_SINGLE ? 4 : 8; }   // x0/xf/dt/tf/F/odata/traj t
+    int observerId() const;
+    ProgramConfig programConfig() const;
+    std::shared_ptr<Program> currentProgram();
+    void ensureStream();
+    void setNpts(int n);
+    void upload(DeviceBuffer &b, const std::vector<double> &v, size_t elem);
+    std::vector<double> download(const DeviceBuffer &b, size_t n, size_t elem);
+    ClodeArgs makeArgs();
+    void launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads);
+    void updateObserverInfo();
+    void resizeFeaturesVariables();
+    void resizeTrajectoryVariables();
+    void requireData(const char *what) const;
+};
+
+}  // namespace clode

@@ -1,0 +1,206 @@
+/*
+ * clode_hip.h - C ABI of the MI355X ensemble integrator (libclode_hip.so).
+ *
+ * One opaque handle per simulator, plain pointers and sizes, an int status on
+ * every call and a thread-local last-error string.  The entry points replace
+ * the C++ classes that the reference's bindings call; each declaration names
+ * the reference member it stands in for.  Reference paths are relative to
+ * the clODE repository (patrickfletcher/clODE):
+ *   clode/cpp/CLODE.hpp, CLODE.cpp                 (base simulator)
+ *   clode/cpp/CLODEfeatures.hpp, CLODEfeatures.cpp (feature simulator)
+ *   clode/cpp/CLODEtrajectory.hpp, CLODEtrajectory.cpp
+ *   clode/cpp/CLODEpython.cpp                      (pybind11 module)
+ *   matlab/clODEmex.cpp, clODEfeaturesmex.cpp, clODEtrajectorymex.cpp
+ *
+ * Arrays cross the boundary as double in the reference's column-major
+ * structure-of-arrays layout: x0/xf [nVar][nPts], pars [nPar][nPts],
+ * F [nFeatures][nPts], t [maxStore][nPts], x/dx [maxStore][nVar][nPts],
+ * aux [maxStore][nAux][nPts] (1 element when nAux == 0), nStored [nPts].
+ *
+ * Status codes: CLODE_OK, CLODE_ERR_INVALID (bad argument; the reference
+ * throws std::invalid_argument -> Python ValueError), CLODE_ERR_RUNTIME
+ * (device / compiler failure; the reference rethrows cl::Error -> Python
+ * RuntimeError).  clode_last_error() describes the last failure on the
+ * calling thread.
+ */
+#ifndef CLODE_HIP_H
+#define CLODE_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CLODE_OK 0
+#define CLODE_ERR_INVALID 1
+#define CLODE_ERR_RUNTIME 2
+
+/* precision modes: SINGLE and DOUBLE follow clode/cpp/realtype.cl; FP32C is
+ * float state with Kahan-compensated updates and double time/error norm */
+#define CLODE_PREC_SINGLE 1
+#define CLODE_PREC_DOUBLE 2
+#define CLODE_PREC_FP32C 3
+
+/* simulator kinds: CLODE, CLODEfeatures, CLODEtrajectory */
+#define CLODE_KIND_BASE 0
+#define CLODE_KIND_FEATURES 1
+#define CLODE_KIND_TRAJECTORY 2
+
+typedef struct clode_sim clode_sim;
+
+/* SolverParams<double> (clode/cpp/clODE_struct_defs.cl:11-20) */
+typedef struct clode_solver_params {
+    double dt;
+    double dtmax;
+    double abstol;
+    double reltol;
+    unsigned int max_steps;
+    unsigned int max_store;
+    unsigned int nout;
+} clode_solver_params;
+
+/* ObserverParams<double> (clode/cpp/observers.cl:25-46) */
+typedef struct clode_observer_params {
+    unsigned int eVarIx;
+    unsigned int fVarIx;
+    unsigned int maxEventCount;
+    unsigned int maxEventTimestamps;
+    double minXamp;
+    double minIMI;
+    double nHoodRadius;
+    double xUpThresh;
+    double xDownThresh;
+    double dxUpThresh;
+    double dxDownThresh;
+    double eps_dx;
+} clode_observer_params;
+
+/* ProblemInfo (clode/cpp/CLODE.hpp:22-91); names are '\n'-separated */
+typedef struct clode_problem_info {
+    const char *src_file;
+    int num_var;
+    int num_par;
+    int num_aux;
+    int num_noise;
+    const char *var_names;
+    const char *par_names;
+    const char *aux_names;
+} clode_problem_info;
+
+/* device description (stands in for OpenCLResource / deviceInfo,
+ * clode/cpp/OpenCLResource.hpp:33-71) */
+typedef struct clode_device_info {
+    char name[256];
+    char arch[64];
+    int compute_units;
+    int max_clock_khz;
+    int max_work_group_size;
+    unsigned long long device_memory_size;
+    unsigned long long max_memory_alloc_size;
+    int double_support;
+} clode_device_info;
+
+const char *clode_last_error(void);
+const char *clode_version(void);
+
+/* ---- devices (OpenCLResource ctor + getDeviceInfo, OpenCLResource.cpp:20-128,367-419) */
+int clode_device_count(int *count);
+int clode_device_query(int device, clode_device_info *info);
+
+/* ---- lifetime (CLODE::CLODE CLODE.cpp:13-37; CLODEfeatures::CLODEfeatures
+ *      CLODEfeatures.cpp:9-29; CLODEtrajectory::CLODEtrajectory CLODEtrajectory.cpp:6-20).
+ *      observer/op are used by CLODE_KIND_FEATURES only (may be NULL otherwise).
+ *      clode_root: directory for the on-disk code-object cache (may be NULL). */
+int clode_create(clode_sim **out, int kind, const clode_problem_info *pi, const char *stepper,
+                 const char *observer, const clode_observer_params *op, int precision, int device,
+                 const char *clode_root);
+int clode_destroy(clode_sim *sim);
+
+/* ---- setters that change the program (CLODE.hpp:148-154) */
+int clode_set_problem_info(clode_sim *sim, const clode_problem_info *pi);
+int clode_set_stepper(clode_sim *sim, const char *stepper);     /* unknown name: warning, unchanged */
+int clode_set_precision(clode_sim *sim, int precision);
+int clode_set_device(clode_sim *sim, int device);                /* setOpenCL */
+int clode_build(clode_sim *sim);                                 /* buildCL */
+
+/* ---- data (CLODE.hpp:157-164) */
+int clode_set_problem_data(clode_sim *sim, const double *x0, size_t nx0, const double *pars, size_t npars);
+int clode_set_tspan(clode_sim *sim, const double *tspan, size_t n);
+int clode_set_x0(clode_sim *sim, const double *x0, size_t n);
+int clode_set_pars(clode_sim *sim, const double *pars, size_t n);
+int clode_set_solver_params(clode_sim *sim, const clode_solver_params *sp);
+int clode_seed_rng(clode_sim *sim);                 /* seedRNG(): random_device */
+int clode_seed_rng_int(clode_sim *sim, int seed);   /* seedRNG(int) */
+
+/* ---- simulation (CLODE.hpp:167-174) */
+int clode_transient(clode_sim *sim);
+int clode_shift_tspan(clode_sim *sim);
+int clode_shift_x0(clode_sim *sim);
+
+/* ---- getters (CLODE.hpp:177-188); *_n return element counts */
+int clode_get_n_pts(clode_sim *sim, int *n);
+int clode_get_tspan(clode_sim *sim, double *tspan2);
+int clode_get_solver_params(clode_sim *sim, clode_solver_params *sp);
+int clode_get_pars(clode_sim *sim, double *out, size_t n);
+int clode_get_x0(clode_sim *sim, double *out, size_t n);
+int clode_get_xf(clode_sim *sim, double *out, size_t n);
+int clode_get_dt(clode_sim *sim, double *out, size_t n);
+int clode_get_tf(clode_sim *sim, double *out, size_t n);
+/* string getters: copy up to cap bytes (NUL-terminated), *needed = full length + 1 */
+int clode_get_available_steppers(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+int clode_get_program_string(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+int clode_get_problem_info(clode_sim *sim, int *nvar, int *npar, int *naux, int *nnoise);
+int clode_get_precision(clode_sim *sim, int *precision);
+int clode_get_stepper(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+int clode_print_status(clode_sim *sim);
+
+/* ---- features (CLODEfeatures.hpp:54-75) */
+int clode_set_observer_params(clode_sim *sim, const clode_observer_params *op);
+int clode_set_observer(clode_sim *sim, const char *observer);   /* unknown name: warning, unchanged */
+int clode_initialize_observer(clode_sim *sim);
+int clode_is_observer_initialized(clode_sim *sim, int *initialized);
+int clode_features(clode_sim *sim, int reinitialize);          /* reinitialize: -1 = features(), 0/1 = features(bool) */
+int clode_get_observer_params(clode_sim *sim, clode_observer_params *op);
+int clode_get_observer_name(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+int clode_get_n_features(clode_sim *sim, int *n);
+int clode_get_f(clode_sim *sim, double *out, size_t n);
+int clode_get_feature_names(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+int clode_get_available_observers(clode_sim *sim, char *buf, size_t cap, size_t *needed);
+
+/* ---- trajectory (CLODEtrajectory.hpp:36-50) */
+int clode_trajectory(clode_sim *sim);
+int clode_get_trajectory_sizes(clode_sim *sim, size_t *nt, size_t *nx, size_t *naux);
+int clode_get_t(clode_sim *sim, double *out, size_t n);
+int clode_get_x(clode_sim *sim, double *out, size_t n);
+int clode_get_dx(clode_sim *sim, double *out, size_t n);
+int clode_get_aux(clode_sim *sim, double *out, size_t n);
+int clode_get_n_stored(clode_sim *sim, int *out, size_t n);
+
+/* ---- logging (CLODEpython.cpp:43-95 LogLevel / get_logger):
+ *      levels 0 trace, 1 debug, 2 info, 3 warn, 4 err, 5 critical, 6 off */
+int clode_set_log_level(int level);
+int clode_get_log_level(int *level);
+
+/* ---- extensions with no reference counterpart ----------------------- */
+/* Work distribution for features/transient launches: mode 0 = one member per
+ * lane in a 1-D grid (the reference's NDRange), 1 = persistent grid whose
+ * lanes claim members from an atomic work counter; block_size 0 = auto. */
+int clode_set_launch(clode_sim *sim, int mode, int block_size);
+/* Per-member end times (uneven spans, persistent mode): tspan_m is [2][nPts]
+ * (t0 row then t1 row) or NULL to use the global span again. */
+int clode_set_member_tspan(clode_sim *sim, const double *tspan_m, size_t n);
+/* Refill the device step-size buffer with sp.dt on set_solver_params (the
+ * reference refills it only when nPts changes). */
+int clode_set_dt_refresh(clode_sim *sim, int enable);
+/* Time of the last kernel launch on the simulator's stream, milliseconds
+ * (HIP events around the launch). */
+int clode_last_kernel_ms(clode_sim *sim, double *ms);
+/* Compile-only check of the current configuration (no device needed). */
+int clode_compile(clode_sim *sim);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CLODE_HIP_H */
