@@ -53,6 +53,7 @@ struct ClodeArgs {
     unsigned int *work_counter;  // next member to claim (persistent kernel)
     void *tspan_m;               // optional per-member [2][nPts] time spans (double)
     unsigned int *status;        // optional per-member status word
+    unsigned long long *steps;   // optional per-member step counter (accumulated)
     unsigned int member_offset;  // first member handled by this launch
     unsigned int n_launch;       // members handled by this launch
 };

@@ -154,6 +154,13 @@ CLODE_DEV void store_member(Member &m, const ClodeArgs &a, unsigned int i)
     ((dtstore *)a.tf)[i] = m.ti;
 }
 
+// optional per-member accumulation of member-steps (benchmark accounting)
+CLODE_DEV void count_steps(const ClodeArgs &a, unsigned int i, unsigned int step)
+{
+    if (a.steps)
+        a.steps[i] += step;
+}
+
 #if CLODE_PREC == 3
 #define CLODE_STEP(m)                                                                              \
     stepper(m.ti, m.xi, m.xc, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd, a.sp_reltol,            \
@@ -180,6 +187,7 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_transient(co
         CLODE_STEP(m);
     }
     store_member(m, a, i);
+    count_steps(a, i, step);
 }
 
 #ifdef CLODE_OBSERVER
@@ -210,6 +218,7 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_initialize_o
     m.ti = t0;
     load_state(m, a, i);
     getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
+    count_steps(a, i, step);
 #endif
     ob.initEventDetector(m.ti, m.xi, m.dxi, m.auxi, op);
     ObsStore st{io};
@@ -252,6 +261,7 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_features(con
     ObsStore st{io};
     ob.fields(st);
     store_member(m, a, i);
+    count_steps(a, i, step);
 }
 #endif  // CLODE_OBSERVER
 
@@ -298,6 +308,7 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_trajectory(c
     }
     a.nStored[i] = (int)storeix;
     store_member(m, a, i);
+    count_steps(a, i, step);
 }
 
 #endif  // CLODE_KERNELS_H

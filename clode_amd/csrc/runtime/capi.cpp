@@ -557,5 +557,24 @@ int clode_last_kernel_ms(clode_sim *sim, double *ms)
     CHECK_SIM(sim) CLODE_TRY *ms = sim->s->lastKernelMs;
     CLODE_CATCH
 }
+int clode_set_step_counter(clode_sim *sim, int enable)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setStepCounter(enable != 0);
+    CLODE_CATCH
+}
+int clode_reset_step_counts(clode_sim *sim)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->resetStepCounts();
+    CLODE_CATCH
+}
+int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    std::vector<unsigned long long> v = sim->s->getStepCounts();
+    if (n < v.size())
+        throw InvalidArgument("output buffer too small");
+    std::copy(v.begin(), v.end(), out);
+    CLODE_CATCH
+}
 
 }  // extern "C"

@@ -248,6 +248,7 @@ void Simulator::setNpts(int newNpts)
     upload(d_dt, dtHost, wideSize());
     memberTspan = false;
     d_tspan_m.release();
+    d_steps.release();
     seedRNG();
     log_msg(LOG_DEBUG, "set nPts=" + std::to_string(nPts));
 }
@@ -416,6 +417,13 @@ ClodeArgs Simulator::makeArgs()
     a.tr_aux = d_aux.ptr;
     a.nStored = d_nStored.as<int>();
     a.tspan_m = memberTspan ? d_tspan_m.ptr : nullptr;
+    if (stepCounter) {
+        if (!d_steps.ptr || d_steps.bytes < (size_t)nPts * 8) {
+            d_steps.allocate(device, (size_t)nPts * 8);
+            resetStepCounts();
+        }
+        a.steps = d_steps.as<unsigned long long>();
+    }
     a.member_offset = 0;
     a.n_launch = (unsigned int)nPts;
     return a;
@@ -438,6 +446,34 @@ void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThr
     float ms = 0.f;
     hipEventElapsedTime(&ms, ev0, ev1);
     lastKernelMs = ms;
+}
+
+void Simulator::setStepCounter(bool on)
+{
+    stepCounter = on;
+    if (!on)
+        d_steps.release();
+}
+
+void Simulator::resetStepCounts()
+{
+    if (!d_steps.ptr)
+        return;
+    ensureStream();
+    hip_check(hipMemsetAsync(d_steps.ptr, 0, d_steps.bytes, stream), "hipMemset");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+}
+
+std::vector<unsigned long long> Simulator::getStepCounts()
+{
+    std::vector<unsigned long long> out(nPts > 0 ? nPts : 0, 0ull);
+    if (!d_steps.ptr || out.empty())
+        return out;
+    ensureStream();
+    hip_check(hipMemcpyAsync(out.data(), d_steps.ptr, out.size() * 8, hipMemcpyDeviceToHost, stream),
+              "hipMemcpy D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return out;
 }
 
 // CLODE::transient (CLODE.cpp:468-493)

@@ -94,12 +94,17 @@ public:
     int blockSize = 0;    // 0 = auto
     bool dtRefresh = false;
     double lastKernelMs = 0.0;
+    bool stepCounter = false;
+
+    void setStepCounter(bool on);
+    void resetStepCounts();
+    std::vector<unsigned long long> getStepCounts();
 
 private:
     DeviceBuffer d_x0, d_pars, d_xf, d_rng, d_dt, d_tf;
     DeviceBuffer d_odr, d_odu, d_F;
     DeviceBuffer d_t, d_x, d_dx, d_aux, d_nStored;
-    DeviceBuffer d_work, d_tspan_m;
+    DeviceBuffer d_work, d_tspan_m, d_steps;
     size_t odrFields = 0, oduFields = 0, Felements = 0;
     bool memberTspan = false;
     hipStream_t stream = nullptr;

@@ -198,6 +198,11 @@ int clode_set_dt_refresh(clode_sim *sim, int enable);
 int clode_last_kernel_ms(clode_sim *sim, double *ms);
 /* Compile-only check of the current configuration (no device needed). */
 int clode_compile(clode_sim *sim);
+/* Per-member count of accepted steps (member-steps), accumulated over every
+ * launch while enabled: transient, observer warm-up, features, trajectory. */
+int clode_set_step_counter(clode_sim *sim, int enable);
+int clode_reset_step_counts(clode_sim *sim);
+int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n);
 
 #ifdef __cplusplus
 }

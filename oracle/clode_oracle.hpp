@@ -127,6 +127,15 @@ inline R randn(RngData<R> &rd)
     return y1;
 }
 
+// fixed-capacity scratch vector (no allocation in the step loop)
+constexpr int MAXV = 64;
+template <class R>
+struct Buf {
+    R v[MAXV];
+    R *data() { return v; }
+    R &operator[](int i) { return v[i]; }
+};
+
 // --------------------------------------------------------------- problem
 // The right-hand side is supplied by the model translation unit as
 // getRHS(t, x, p, dx, aux, w) overloads for float and double.
@@ -156,7 +165,7 @@ struct Integrator {
     void heun_step(R *ti, R *xi, R *k1, const R *p, R dt, R *aux, const R *wi) const
     {
         R th = *ti + dt;
-        std::vector<R> tmp(pb.nVar), k2(pb.nVar);
+        Buf<R> tmp, k2;
         for (int k = 0; k < pb.nVar; k++)
             tmp[k] = std::fma(dt, k1[k], xi[k]);
         rhs(th, tmp.data(), p, k2.data(), aux, wi);
@@ -168,7 +177,7 @@ struct Integrator {
     void rk4_step(R *ti, R *xi, R *k1, const R *p, R dt, R *aux, const R *wi) const
     {
         const int N = pb.nVar;
-        std::vector<R> tmp(N), k2(N), k3(N), k4(N);
+        Buf<R> tmp, k2, k3, k4;
         R h2 = dt * OC(0.5);
         R th2 = *ti + h2;
         R th = *ti + dt;
@@ -211,7 +220,7 @@ struct Integrator {
                 E4 = OC(-1.0) / OC(8.0);
         R tNew = *ti + dt;
         R newDt = tNew - *ti;
-        std::vector<R> xtmp(N), k2(N), k3(N), k4(N);
+        Buf<R> xtmp, k2, k3, k4;
         for (int k = 0; k < N; k++)
             xtmp[k] = xi[k] + newDt * OC(0.5) * k1[k];
         rhs(*ti + newDt * OC(0.5), xtmp.data(), p, k2.data(), aux, wi);
@@ -247,7 +256,7 @@ struct Integrator {
                 E5 = OC(-17253.0) / OC(339200.0), E6 = OC(22.0) / OC(525.0), E7 = OC(-1.0) / OC(40.0);
         R tNew = *ti + dt;
         R newDt = tNew - *ti;
-        std::vector<R> xtmp(N), k2(N), k3(N), k4(N), k5(N), k6(N), k7(N);
+        Buf<R> xtmp, k2, k3, k4, k5, k6, k7;
         for (int k = 0; k < N; k++)
             xtmp[k] = xi[k] + newDt * (B21 * k1[k]);
         rhs(*ti + A2 * newDt, xtmp.data(), p, k2.data(), aux, wi);
@@ -285,7 +294,7 @@ struct Integrator {
         const R maxGrow = OC(5.0);
         const R expon = OC(1.0) / (order + OC(1.0));
         R tNew, normErr;
-        std::vector<R> err(N), newxi(N), newk1(N);
+        Buf<R> err, newxi, newk1;
         R newDt = *dt;
         R threshold = sp.abstol / sp.reltol;
         R hmin = OC(16.0) * std::fabs(std::fabs(std::nextafter(*ti, OC(1.1) * t1)) - *ti);
@@ -641,7 +650,7 @@ struct Observer {
         } else {  // nhood2 :157-175
             if (od.stepcount < 2 || !od.foundX0)
                 return false;
-            std::vector<R> d(N);
+            Buf<R> d;
             for (int j = 0; j < N; ++j)
                 d[j] = (xi[j] - od.x0[j]) / od.xTrajectoryRange[j];
             unsigned int lastInNhood = od.isInNhood;

@@ -430,6 +430,8 @@ extern "C" {
 void *oracle_create(int kind, int prec, int stepper, int observer, int nVar, int nPar, int nAux, int nWiener,
                     int nStore)
 {
+    if (nVar <= 0 || nVar > oracle::MAXV)
+        return nullptr;
     Handle *H = new Handle;
     H->cfg = Config{kind, prec, stepper, observer, nVar, nPar, nAux, nWiener, nStore, 1};
     if (prec == 1)
