@@ -199,7 +199,19 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-members", type=int, default=1024)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="run only a 256 MiB device copy kernel (FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        import torch
+
+        x = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)

@@ -10,7 +10,7 @@
 // sqrt, rint, ldexp, frexp) and fixed constants only, so they return the same
 // bits wherever they are compiled (hipRTC for gfx950, clang++ for x86-64 with
 // FMA).  Accuracy against glibc is checked in the test-suite (<= 2 ulp for
-// exp/log, <= 16 ulp for pow, the OpenCL C 1.2 bound for pow/pown).
+// exp, log and pow; the OpenCL C 1.2 bounds are 3, 3 and 16 ulp).
 //
 // Methods:
 //  exp: Cody-Waite argument reduction x = k ln2 + r, |r| <= ln2/2, followed by
@@ -19,8 +19,10 @@
 //  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f);
 //       log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))) where R is the series
 //       2z/3 + 2z^2/5 + ... of 2 atanh(s)/s - 2 (11 terms double, 4 float).
-//  pow: IEEE-754 special cases, then exp(y log|x|) with the sign of x for odd
-//       integer y.
+//  pow: IEEE-754 special cases, then exp(p) (1 + q) where p + q = y log|x| is
+//       formed in double-double (log from the atanh series with exact two-sums
+//       and fma-based products), with the sign of x for odd integer y; the
+//       float version rounds the double result once.
 //  pown: binary exponentiation (exact operation sequence).
 //
 // CM_FN is defined by the including translation unit (host or device).
@@ -103,6 +105,52 @@ CM_FN double log(double x)
     return de * ln2hi - ((hfsq - lo) - f);
 }
 
+// log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)), x finite and > 0:
+// log x = e ln2 + 2 atanh(s), s = (m-1)/(m+1), with s, m+1 and the leading
+// terms carried in double-double (Dekker products / Knuth two-sums via fma).
+CM_FN double log_dd(double x, double *lo)
+{
+    int e;
+    double m = frexp(x, &e);
+    if (m < 0x1.6a09e667f3bcdp-1) {
+        m = m * 2.0;
+        e = e - 1;
+    }
+    double num = m - 1.0;             // exact
+    double den = m + 1.0;             // two-sum: den + den_lo = m + 1 exactly
+    double dbb = den - m;
+    double den_lo = (m - (den - dbb)) + (1.0 - dbb);
+    double s = num / den;
+    double r = fma(-s, den, num);     // num - s*den exactly
+    r = fma(-s, den_lo, r);
+    double s_lo = r / den;
+    double z = s * s;
+    double T = 0x1.642c8590b2164p-4;
+    T = fma(T, z, 0x1.8618618618618p-4);
+    T = fma(T, z, 0x1.af286bca1af28p-4);
+    T = fma(T, z, 0x1.e1e1e1e1e1e1ep-4);
+    T = fma(T, z, 0x1.1111111111111p-3);
+    T = fma(T, z, 0x1.3b13b13b13b14p-3);
+    T = fma(T, z, 0x1.745d1745d1746p-3);
+    T = fma(T, z, 0x1.c71c71c71c71cp-3);
+    T = fma(T, z, 0x1.2492492492492p-2);
+    T = fma(T, z, 0x1.999999999999ap-2);
+    T = fma(T, z, 0x1.5555555555555p-1);
+    T = T * z * s;                    // 2 atanh(s) = 2s + T
+    const double ln2hi = 0x1.62e42fee00000p-1;
+    const double ln2lo = 0x1.a39ef35793c76p-33;
+    double de = (double)e;
+    double a = de * ln2hi;            // exact
+    double b = 2.0 * s;               // exact
+    double hi = a + b;                // two-sum
+    double bb = hi - a;
+    double err = (a - (hi - bb)) + (b - bb);
+    double l = err + 2.0 * s_lo + T + de * ln2lo;
+    double h2 = hi + l;               // renormalise
+    *lo = l - (h2 - hi);
+    return h2;
+}
+
 CM_FN double pow(double x, double y)
 {
     if (y == 0.0 || x == 1.0)
@@ -129,7 +177,15 @@ CM_FN double pow(double x, double y)
         double r = (y > 0.0) ? __builtin_huge_val() : 0.0;
         return (x < 0.0 && yodd) ? -r : r;
     }
-    double r = cm::exp(y * cm::log(ax));
+    // y log|x| in double-double, then exp(p + q) = exp(p) (1 + q)
+    double llo;
+    double lhi = cm::log_dd(ax, &llo);
+    double p = y * lhi;
+    double q = fma(y, lhi, -p);
+    q = fma(y, llo, q);
+    double r = cm::exp(p);
+    if (r != 0.0 && r != __builtin_huge_val())
+        r = fma(r, q, r);
     return (x < 0.0 && yodd) ? -r : r;
 }
 
@@ -230,7 +286,8 @@ CM_FN float pow(float x, float y)
         float r = (y > 0.0f) ? __builtin_huge_valf() : 0.0f;
         return (x < 0.0f && yodd) ? -r : r;
     }
-    float r = cm::exp(y * cm::log(ax));
+    // evaluated in double: one final rounding to float
+    float r = (float)cm::pow((double)ax, (double)y);
     return (x < 0.0f && yodd) ? -r : r;
 }
 

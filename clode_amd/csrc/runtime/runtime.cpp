@@ -203,7 +203,13 @@ std::string compose_source(const ProgramConfig &cfg)
     s << "#define N_STORE_EVENTS " << cfg.nStore << "\n";
     s << "#include \"clode_prelude.h\"\n";
     s << "#pragma clang force_cuda_host_device begin\n";
-    s << "#line 1 \"" << (cfg.rhs_name.empty() ? std::string("rhs.cl") : cfg.rhs_name) << "\"\n";
+    // basename only: the program text (and its cache key) must not depend on
+    // where the model file lives
+    std::string rname = cfg.rhs_name.empty() ? std::string("rhs.cl") : cfg.rhs_name;
+    size_t slash = rname.find_last_of('/');
+    if (slash != std::string::npos)
+        rname = rname.substr(slash + 1);
+    s << "#line 1 \"" << rname << "\"\n";
     s << cfg.rhs << "\n";
     s << "#pragma clang force_cuda_host_device end\n";
     s << "#include \"clode_rng.h\"\n";

@@ -1,0 +1,109 @@
+"""The reference's own Python tests, run through clode_amd's Python API on the
+GPU (test/test_vdp.py, test/test_features.py, test/test_aux_values.py,
+test/test_ornl_thompson_a1.py), plus ensemble construction semantics of
+clode/solver.py:254-462."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+import clode_amd as clode
+
+pytestmark = pytest.mark.gpu
+
+MODELS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "clode_amd", "models")
+
+
+def vdp_period(mu):
+    if mu < 0:
+        return 0.0
+    if mu < 2:
+        return 2 * math.pi * (1 + mu ** 2 / 16)
+    return min(2 * math.pi * (1 + mu ** 2 / 16), (3 - 2 * math.log(2)) * mu + 3 * 2.2338 / mu ** (1 / 3))
+
+
+@pytest.mark.parametrize("precision", [True, False, "fp32c"])
+def test_vdp_dormand_prince(gpu_available, precision):
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"), variables={"x": 1.0, "y": 1.0},
+                                 parameters={"mu": 1.0}, observer=clode.Observer.threshold_2,
+                                 stepper=clode.Stepper.dormand_prince, t_span=(0.0, 1000.0), max_store=20000,
+                                 max_steps=20000, single_precision=precision)
+    mus = [-1, 0, 0.01, 0.1, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0] + list(range(5, 7))
+    sim.set_ensemble(parameters={"mu": mus})
+    sim.features()
+    periods = sim.get_observer_results().get_var_max("period")
+    for i, mu in enumerate(mus):
+        assert np.isclose(periods[i], vdp_period(mu), rtol=0.01, atol=1), (mu, periods[i])
+
+
+def test_sine_curve_timestamps(gpu_available):
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "sine.cl"), variables={"x": 0.0},
+                                 parameters={"omega": 1.0}, aux=["xp1", "pos", "neg"],
+                                 observer=clode.Observer.threshold_2, stepper=clode.Stepper.rk4, dtmax=0.001,
+                                 dt=0.001, t_span=(0.0, 4 * math.pi), event_var="x", feature_var="x",
+                                 observer_min_x_amp=0.5, observer_x_up_thresh=(2 + math.sqrt(2)) / 4,
+                                 observer_x_down_thresh=0.001, observer_dx_down_thresh=0.001,
+                                 observer_dx_up_thresh=0.001, observer_max_event_count=100,
+                                 observer_max_event_timestamps=3)
+    out = sim.features()
+    assert int(out.get_var_count("event")) == 2
+    up, down = out.get_event_data("up"), out.get_event_data("down")
+    assert len(up) == 2 and len(down) == 2
+    assert np.isclose(up[0], math.pi / 4, atol=0.01) and np.isclose(up[1], 9 * math.pi / 4, atol=0.01)
+    assert np.isclose(down[0], 3 * math.pi / 2, atol=0.01) and np.isclose(down[1], 7 * math.pi / 2, atol=0.01)
+    assert list(out.get_timestamps("up")) == list(up)
+
+
+def test_aux_values(gpu_available):
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "sine.cl"), variables={"x": 0.0},
+                                 parameters={"omega": 1.0}, aux=["xp1", "pos", "neg"],
+                                 observer=clode.Observer.threshold_2, stepper=clode.Stepper.rk4, dtmax=0.1, dt=0.1,
+                                 t_span=(0.0, 400 * math.pi), event_var="x", feature_var="x")
+    out = sim.features()
+    assert out.get_var_mean("xp1") == pytest.approx(1.0, abs=1e-2)
+    assert out.get_var_min("xp1") == pytest.approx(0.0, abs=1e-2)
+    assert out.get_var_max("xp1") == pytest.approx(2.0, abs=1e-2)
+    assert out.get_var_mean("pos") == 1.0 and out.get_var_min("pos") == 1.0 and out.get_var_max("pos") == 1.0
+    assert out.get_var_min("neg") == -2.0 and out.get_var_max("neg") == -2.0
+
+
+def test_ornl_thompson_a1(gpu_available):
+    m, w, k, H = 1 / 4, 8.0, 2.0, 10.0
+    sim = clode.TrajectorySimulator(src_file=os.path.join(MODELS, "linear_drag.cl"), variables={"y1": 0.0, "y2": 0.0},
+                                    parameters={"m": m, "w": w, "k": k, "H": H}, aux=["g1"], num_noise=0, dt=0.001,
+                                    dtmax=0.001, stepper=clode.Stepper.rk4, t_span=(0.0, H / 2.0), max_store=20000,
+                                    max_steps=20000)
+    traj = sim.trajectory()
+    for tt, y1, y2 in zip(traj.t[1:], traj.x["y1"][1:], traj.x["y2"][1:]):
+        np.testing.assert_approx_equal(y1, 4.0 * (tt + 1.0 / 8.0 * math.exp(-8.0 * tt) - 1.0 / 8.0), significant=5)
+        np.testing.assert_approx_equal(y2, 4.0 * (1.0 - math.exp(-8.0 * tt)), significant=5)
+    assert np.allclose(traj.aux["g1"], traj.x["y1"] - H)
+
+
+def test_ensemble_construction_semantics(gpu_available):
+    sim = clode.Simulator(src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 2.0, "z": 3.0},
+                          parameters={"sigma": 10.0, "rho": 28.0, "beta": 8 / 3}, stepper=clode.Stepper.rk4,
+                          dt=0.001, t_span=(0.0, 1.0), single_precision=False)
+    sim.set_ensemble(parameters={"rho": np.linspace(0, 30, 12).reshape(3, 4)})
+    assert sim._ensemble_size == 12 and sim._ensemble_shape == (3, 4)
+    x0 = sim.get_initial_state()
+    assert x0.shape == (12, 3) and np.all(x0 == [1.0, 2.0, 3.0])
+    with pytest.raises(ValueError):
+        sim.set_ensemble(parameters={"nope": 1.0})
+    with pytest.raises(ValueError):
+        sim.set_ensemble(variables={"x": [1, 2]}, parameters={"rho": [1, 2, 3]})
+    with pytest.raises(ValueError):
+        sim.set_ensemble()
+    xf = sim.transient(fetch_results=True)
+    assert xf.shape == (12, 3) and np.all(np.isfinite(xf))
+    assert np.array_equal(sim.get_initial_state(), xf)  # update_x0 shifts xf -> x0
+    assert sim.get_final_time().shape == (3, 4)
+    sim.set_repeat_ensemble(5)
+    assert sim.get_initial_state().shape == (5, 3)
+    sim.seed_rng(1)
+    sim.set_tspan((0.0, 0.5))
+    assert sim.get_tspan() == (0.0, 0.5)
+    sim.shift_tspan()
+    assert sim.get_tspan() == (0.5, 1.0)
+    assert "rk4" in sim.get_available_steppers()

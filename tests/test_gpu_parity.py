@@ -1,0 +1,220 @@
+"""GPU parity: the HIP kernels (through the C ABI via the pybind11 module)
+against the CPU oracle on the same inputs, bit for bit.
+
+Plain single and double precision run the reference's arithmetic with the
+same fused multiply-adds and the same deterministic exp/log/pow on both
+sides, so every state, step size, time, feature and RNG word must be equal,
+not merely close.  Ensemble sizes are deliberately not multiples of the
+64-lane wavefront.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from helpers import Pair
+
+pytestmark = pytest.mark.gpu
+
+FIXED = (0.01, 1.0, 1e-6, 1e-3, 1000000, 1000000, 1)
+ADAPT = (0.1, 1.0, 1e-8, 1e-6, 10000000, 1000000, 1)
+
+
+def vdp_data(n):
+    mu = np.linspace(0.1, 5.0, n)
+    return np.vstack([2.0 * np.ones(n), np.zeros(n)]), mu[None, :]
+
+
+def lac_data(n, noise=False):
+    g = np.linspace(0.0, 10.0, n)
+    b = np.linspace(2.0, 0.0, n)
+    x0 = np.tile(np.array([[-60.0], [0.0], [0.0], [0.1]]), (1, n))
+    last = np.full(n, 2.0) if noise else g  # sigma or gk
+    pars = np.vstack([1.5 * np.ones(n), 3.0 * np.ones(n), b, last])
+    return x0, pars
+
+
+def lorenz_data(n, seed=3):
+    rng = np.random.default_rng(seed)
+    return rng.normal(1.0, 0.1, (3, n)), np.vstack([10.0 * np.ones(n), np.linspace(0, 250, n), np.full(n, 8 / 3)])
+
+
+CASES = []
+for prec in ("double", "single"):
+    for stepper in ("euler", "heun", "rk4"):
+        for observer in ("thresh2", "localmax", "nhood2"):
+            CASES.append(("van_der_pol", stepper, observer, prec, 97, (0.0, 60.0), FIXED))
+    for stepper in ("bs23", "dopri5"):
+        CASES.append(("lactotroph", stepper, "thresh2", prec, 70, (0.0, 1500.0), ADAPT))
+        CASES.append(("lorenz", stepper, "localmax", prec, 130, (0.0, 40.0), ADAPT))
+        CASES.append(("lactotroph", stepper, "nhood2", prec, 33, (0.0, 1000.0), ADAPT))
+    for observer in ("thresh2", "localmax", "nhood2"):
+        CASES.append(("lactotroph_noise", "seuler", observer, prec, 65, (0.0, 400.0),
+                      (0.05, 1.0, 1e-6, 1e-3, 1000000, 1000000, 1)))
+
+
+def data_for(model, n):
+    if model == "van_der_pol":
+        return vdp_data(n)
+    if model == "lactotroph":
+        return lac_data(n)
+    if model == "lactotroph_noise":
+        return lac_data(n, noise=True)
+    return lorenz_data(n)
+
+
+@pytest.mark.parametrize("model,stepper,observer,prec,n,tspan,sp", CASES,
+                         ids=[f"{c[0]}-{c[1]}-{c[2]}-{c[3]}" for c in CASES])
+def test_transient_then_features_bitexact(gpu_available, model, stepper, observer, prec, n, tspan, sp):
+    p = Pair(model, stepper, observer, prec, obs=dict(max_event_timestamps=3), sp=sp)
+    x0, pars = data_for(model, n)
+    p.set_problem_data(x0, pars)
+    p.seed(1234)
+    p.set_tspan(*tspan)
+    p.transient()
+    p.assert_bitexact(("xf", "dt", "tf"))
+    p.shift_x0()
+    p.features()
+    res = p.assert_bitexact(("F", "xf", "dt", "tf"))
+    assert res["F"][1] == n * p.gpu.get_n_features()
+    # the oracle and the device consumed the same noise
+    assert np.array_equal(np.array(p.gpu.get_xf()), p.orc.get("xf"))
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+def test_observer_continuation_and_reinit(gpu_available, prec):
+    """features() twice continues the observer (time-shifted data, counters
+    keep growing); features(True) re-initialises (CLODEfeatures.cpp:215-229)."""
+    p = Pair("van_der_pol", "rk4", "thresh2", prec, obs=dict(max_event_timestamps=4), sp=FIXED)
+    x0, pars = vdp_data(70)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 40.0)
+    p.features()
+    p.shift_x0()
+    p.features()  # continued
+    p.assert_bitexact(("F", "xf"))
+    F1 = np.array(p.gpu.get_f()).reshape(-1, 70)
+    assert p.gpu.is_observer_initialized()
+    p.shift_x0()
+    p.features(True)  # re-initialised
+    p.assert_bitexact(("F", "xf"))
+    F2 = np.array(p.gpu.get_f()).reshape(-1, 70)
+    names = p.gpu.get_feature_names()
+    sc = names.index("step count")
+    assert np.all(F1[sc] > F2[sc])  # continuation accumulates steps, reinit restarts
+
+
+@pytest.mark.parametrize("prec", ["double", "single"])
+@pytest.mark.parametrize("stepper", ["rk4", "dopri5", "seuler"])
+def test_trajectory_bitexact(gpu_available, prec, stepper):
+    model = "lactotroph_noise" if stepper == "seuler" else "lactotroph"
+    sp = (0.05, 1.0, 1e-8, 1e-6, 10000000, 400, 3)  # nout 3, max_store 400
+    p = Pair(model, stepper, "", prec, kind="trajectory", sp=sp)
+    x0, pars = lac_data(77, noise=(model == "lactotroph_noise"))
+    p.set_problem_data(x0, pars)
+    p.seed(99)
+    p.set_tspan(0.0, 300.0)
+    p.trajectory()
+    assert p.assert_trajectory_bitexact() > 77
+    ns = np.array(p.gpu.get_n_stored())
+    assert ns.max() <= 401
+
+
+def test_trajectory_max_store_truncation(gpu_available):
+    """The store loop stops once max_store samples exist; nStored counts the
+    last step even though sample max_store is never written (reference H9)."""
+    sp = (0.01, 1.0, 1e-6, 1e-3, 10000000, 50, 1)
+    p = Pair("van_der_pol", "rk4", "", "double", kind="trajectory", sp=sp)
+    x0, pars = vdp_data(5)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 100.0)
+    p.trajectory()
+    p.assert_trajectory_bitexact()
+    assert np.all(np.array(p.gpu.get_n_stored()) == 50)
+    tf = np.array(p.gpu.get_tf())
+    assert np.allclose(tf, 0.5, atol=1e-9)  # 50 steps of 0.01
+
+
+def test_seeded_noise_stream(gpu_available):
+    p = Pair("ornstein_uhlenbeck", "seuler", "", "double", kind="base",
+             sp=(0.01, 0.01, 1e-6, 1e-3, 1000000, 1000000, 1))
+    n = 300
+    p.set_problem_data(np.zeros((1, n)), np.vstack([np.ones(n), np.ones(n), 0.5 * np.ones(n)]))
+    p.set_tspan(0.0, 5.0)
+    p.seed(2024)
+    p.transient()
+    p.assert_bitexact(("xf",))
+    # identical seed -> identical result; the RNG words after the run agree too
+    xf1 = np.array(p.gpu.get_xf())
+    p.seed(2024)
+    p.gpu.transient()
+    assert np.array_equal(np.array(p.gpu.get_xf()), xf1)
+    p.seed(2025)
+    p.gpu.transient()
+    assert not np.array_equal(np.array(p.gpu.get_xf()), xf1)
+    # OU stationary law: x ~ N(mu, sigma^2/(2 theta)) after t >> 1/theta
+    p.set_tspan(0.0, 20.0)
+    p.seed(7)
+    p.gpu.transient()
+    x = np.array(p.gpu.get_xf())
+    assert abs(x.mean() - 1.0) < 0.08 and abs(x.var() - 0.125) < 0.04
+
+
+def test_dt_refill_semantics(gpu_available):
+    """The device step buffer is refilled with sp.dt only when nPts changes
+    (CLODE.cpp:184-202, 377-403); set_dt_refresh(True) refills on every
+    set_solver_params."""
+    p = Pair("decay", "euler", "", "double", kind="base", sp=(0.1, 1.0, 1e-6, 1e-3, 1000, 1000, 1))
+    p.set_problem_data(np.ones((1, 3)), np.ones((1, 3)))
+    p.set_solver_params((0.05, 1.0, 1e-6, 1e-3, 1000, 1000, 1))
+    assert np.all(np.array(p.gpu.get_dt()) == 0.1)
+    p.gpu.set_dt_refresh(True)
+    p.set_solver_params((0.05, 1.0, 1e-6, 1e-3, 1000, 1000, 1))
+    assert np.all(np.array(p.gpu.get_dt()) == 0.05)
+    p.set_problem_data(np.ones((1, 4)), np.ones((1, 4)))  # nPts change refills
+    assert np.all(np.array(p.gpu.get_dt()) == 0.05)
+
+
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 257])
+def test_ensemble_sizes_and_max_steps(gpu_available, n):
+    sp = (0.01, 1.0, 1e-6, 1e-3, 777, 1000000, 1)  # max_steps truncates
+    p = Pair("van_der_pol", "rk4", "thresh2", "double", sp=sp)
+    x0, pars = vdp_data(n)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 100.0)
+    p.features()
+    p.assert_bitexact(("F", "xf", "tf"))
+    names = p.gpu.get_feature_names()
+    F = np.array(p.gpu.get_f()).reshape(-1, n)
+    assert np.all(F[names.index("step count")] == 777)
+
+
+def test_terminal_event_count(gpu_available):
+    p = Pair("harmonic", "rk4", "localmax", "double", obs=dict(max_event_count=3, max_event_timestamps=5), sp=FIXED)
+    n = 40
+    p.set_problem_data(np.vstack([np.ones(n), np.zeros(n)]), np.linspace(1.0, 3.0, n)[None, :])
+    p.set_tspan(0.0, 100.0)
+    p.features()
+    p.assert_bitexact(("F", "xf", "tf"))
+    names = p.gpu.get_feature_names()
+    F = np.array(p.gpu.get_f()).reshape(-1, n)
+    assert np.all(F[names.index("event count")] == 3)
+    assert np.all(np.array(p.gpu.get_tf()) < 100.0)  # stopped at the 3rd maximum
+    omega = np.linspace(1.0, 3.0, n)
+    # x = cos(omega t) starts at a maximum; detected maxima at 2 pi k / omega, k = 1, 2, 3
+    assert np.allclose(F[names.index("localmax event time 1")], 4 * math.pi / omega, atol=0.02)
+
+
+def test_step_counter_matches_oracle(gpu_available):
+    p = Pair("lactotroph", "dopri5", "thresh2", "double", sp=ADAPT)
+    x0, pars = lac_data(50)
+    p.set_problem_data(x0, pars)
+    p.gpu.set_step_counter(True)
+    p.set_tspan(0.0, 800.0)
+    p.transient()
+    tr = np.array(p.gpu.get_step_counts(), dtype=np.int64).sum()
+    assert tr == p.orc.last_steps
+    p.gpu.reset_step_counts()
+    p.features()
+    total = np.array(p.gpu.get_step_counts(), dtype=np.int64).sum()
+    assert total == p.orc.last_steps + p.orc.last_warm_steps

@@ -1,0 +1,162 @@
+"""CPU-side checks of the native library and its boundary (no GPU needed):
+exported C ABI, pybind11 module surface and defaults (clode/cpp/CLODEpython.cpp),
+observer layouts and feature names, hipRTC compilation of every program
+family for gfx950, and the runtime's argument validation."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(REPO, "clode_amd", "libclode_hip.so")
+HEADER = os.path.join(REPO, "include", "clode_hip.h")
+MODELS = os.path.join(REPO, "clode_amd", "models")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^(?:int|const char \*)\s*(clode_\w+)\(", text, re.M)))
+
+
+def test_c_abi_exports_every_declared_symbol():
+    lib = ctypes.CDLL(LIB)
+    names = declared_functions()
+    assert len(names) > 60
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_c_abi_without_device():
+    lib = ctypes.CDLL(LIB)
+    n = ctypes.c_int(-1)
+    assert lib.clode_device_count(ctypes.byref(n)) == 0 and n.value == 0
+    lib.clode_last_error.restype = ctypes.c_char_p
+    assert lib.clode_destroy(None) == 0
+    assert lib.clode_transient(None) == 1  # CLODE_ERR_INVALID on a null handle
+    assert b"null" in lib.clode_last_error()
+
+
+def test_pybind_defaults_match_reference():
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    sp = w.SolverParams()
+    assert (sp.dt, sp.dtmax, sp.abstol, sp.reltol, sp.max_steps, sp.max_store, sp.nout) == \
+           (0.1, 0.5, 1e-6, 1e-3, 1000000, 1000000, 1)
+    op = w.ObserverParams()
+    assert (op.e_var_ix, op.f_var_ix, op.max_event_count, op.max_event_timestamps) == (0, 0, 100, 0)
+    assert (op.min_amp, op.min_imi, op.nhood_radius, op.x_up_threshold, op.x_down_threshold) == \
+           (0.0, 0.0, 0.05, 0.2, 0.2)
+    pi = w.ProblemInfo("f.cl", ["x", "y"], ["a"])
+    assert pi.num_noise == 1 and pi.num_var == 2 and pi.num_par == 1 and pi.num_aux == 0
+    pi.vars = ["u"]
+    assert pi.num_var == 1
+    for cls in ("SimulatorBase", "FeatureSimulatorBase", "TrajectorySimulatorBase", "OpenCLResource",
+                "CLDeviceType", "CLVendor", "DeviceInfo", "PlatformInfo", "LogLevel"):
+        assert hasattr(w, cls)
+    for fn in ("query_opencl", "_print_opencl", "get_logger"):
+        assert hasattr(w, fn)
+
+
+@pytest.mark.parametrize("observer,nvar,naux,nstore", [
+    ("thresh2", 4, 1, 0), ("thresh2", 2, 0, 3), ("localmax", 3, 0, 2), ("localmax", 3, 1, 0),
+    ("nhood2", 2, 0, 1), ("nhood2", 4, 2, 0)])
+def test_feature_names_and_counts(observer, nvar, naux, nstore):
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    model = {2: "van_der_pol", 3: "lorenz", 4: "lactotroph"}[nvar]
+    src = os.path.join(MODELS, model + ".cl")
+    vars_ = [f"v{j}" for j in range(nvar)]
+    pars = ["p0", "p1", "p2", "p3"][: {2: 1, 3: 3, 4: 4}[nvar]]
+    aux = [f"a{j}" for j in range(naux)]
+    op = w.ObserverParams(max_event_timestamps=nstore)
+    s = w.FeatureSimulatorBase(w.ProblemInfo(src, vars_, pars, aux, 0), "rk4", observer, op, False,
+                               w.OpenCLResource(), "")
+    names = s.get_feature_names()
+    expected = {"thresh2": 18 + 5 * nvar + 3 * naux + 2 * nstore + 5,
+                "localmax": 6 + 5 * nvar + 3 * naux + 4 * nstore + 2,
+                "nhood2": 6 + 7 * nvar + 3 * naux + nstore + 5}[observer]
+    assert s.get_n_features() == expected == len(names) == len(set(names))
+    assert names[-1 if observer == "localmax" else -4] == "step count"
+    assert f"max d{vars_[-1]}/dt" in names
+    if nstore:
+        key = {"thresh2": "up event time 0", "localmax": "localmax event evar 0", "nhood2": "nhood event time 0"}
+        assert key[observer] in names
+
+
+@pytest.mark.parametrize("obs", [1, 2, 3])
+def test_observer_state_layout_matches_host(tmp_path, obs):
+    """Compile the device observer header on the host and count the
+    persistent fields it spills, against clode_layout.h (host allocation)."""
+    exe = tmp_path / "probe"
+    for nv, na in ((1, 0), (2, 1), (4, 3)):
+        subprocess.run(["g++", "-std=c++17", "-DPREC=2", f"-DCLODE_OBSERVER={obs}", f"-DN_VAR={nv}",
+                        f"-DN_AUX={na}", "-DN_STORE_EVENTS=2", os.path.join(REPO, "tests", "layout_probe.cpp"),
+                        "-o", str(exe)], check=True, cwd=os.path.join(REPO, "tests"))
+        r = subprocess.run([str(exe)], capture_output=True, text=True)
+        assert r.returncode == 0, r.stdout
+
+
+@pytest.mark.parametrize("kind,stepper,observer,prec", [
+    ("features", "rk4", "thresh2", "double"), ("features", "dopri5", "thresh2", "fp32c"),
+    ("features", "bs23", "localmax", "single"), ("features", "seuler", "nhood2", "double"),
+    ("features", "heun", "nhood2", "fp32c"), ("features", "euler", "localmax", "fp32c"),
+    ("trajectory", "dopri5", None, "single"), ("trajectory", "rk4", None, "fp32c"),
+    ("base", "bs23", None, "double")])
+def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    pi = w.ProblemInfo(os.path.join(MODELS, "lactotroph_noise.cl"), ["v", "n", "f", "c"],
+                       ["gcal", "gsk", "gbk", "sigma"], ["ica"], 1)
+    dev = w.OpenCLResource()
+    if kind == "features":
+        s = w.FeatureSimulatorBase(pi, stepper, observer, w.ObserverParams(max_event_timestamps=2), prec, dev, "")
+    elif kind == "trajectory":
+        s = w.TrajectorySimulatorBase(pi, stepper, prec, dev, "")
+    else:
+        s = w.SimulatorBase(pi, stepper, prec, dev, "")
+    s.compile()
+    prog = s.get_program_string()
+    assert "--offload-arch=gfx950" in prog and "-packed-fp32-ops" in prog
+    assert f"#define CLODE_PREC {dict(single=1, double=2, fp32c=3)[prec]}" in prog
+
+
+def test_unknown_names_and_bad_input():
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    pi = w.ProblemInfo(os.path.join(MODELS, "van_der_pol.cl"), ["x", "y"], ["mu"], [], 0)
+    with pytest.raises(ValueError):
+        w.SimulatorBase(pi, "rk5", False, w.OpenCLResource(), "")
+    with pytest.raises(ValueError):
+        w.FeatureSimulatorBase(pi, "rk4", "nosuch", w.ObserverParams(), False, w.OpenCLResource(), "")
+    with pytest.raises(ValueError):
+        w.FeatureSimulatorBase(pi, "rk4", "thresh2", w.ObserverParams(e_var_ix=5), False, w.OpenCLResource(), "")
+    with pytest.raises(ValueError):
+        w.SimulatorBase(w.ProblemInfo("/nonexistent.cl", ["x"], []), "rk4", False, w.OpenCLResource(), "")
+    s = w.SimulatorBase(pi, "rk4", False, w.OpenCLResource(), "")
+    s.set_stepper("nosuch")  # reference: warning, stepper unchanged
+    assert "#define CLODE_STEPPER 3" in s.get_program_string()
+    with pytest.raises(RuntimeError):
+        s.set_problem_data([1.0, 2.0], [1.0])  # no device here
+
+
+def test_runtime_validation():
+    """Reference test/test_runtime.py: invalid device-selection combinations."""
+    from clode_amd.runtime import CLDeviceType, CLVendor, initialize_runtime
+
+    with pytest.raises(ValueError):
+        initialize_runtime(CLDeviceType.DEVICE_TYPE_GPU, None, 0, 0, None)
+    with pytest.raises(ValueError):
+        initialize_runtime(None, CLVendor.VENDOR_AMD, 0, 0, None)
+    with pytest.raises(ValueError):
+        initialize_runtime(None, None, 0, 0, [0])
+    with pytest.raises(ValueError):
+        initialize_runtime(None, None, 0, None, None)
+    with pytest.raises(ValueError):
+        initialize_runtime(None, None, None, 0, None)
+    with pytest.raises(ValueError):
+        initialize_runtime(None, None, None, None, [0])
+    assert initialize_runtime(None, None, 0, 1, None).device == 1
+    assert initialize_runtime(None, None, None, None, None).device == 0

@@ -1,0 +1,81 @@
+"""Pins the deterministic elementary functions (clode_amd/csrc/device/cm_math.h)
+shared by the GPU programs and the CPU oracle against glibc/numpy.
+
+exp, log and pow must stay within 2 ulp of the correctly rounded value (the
+OpenCL C 1.2 bounds are 3, 3 and 16 ulp); pown is exact repeated
+multiplication.  IEEE special cases follow C99 Annex F.
+"""
+import numpy as np
+import pytest
+
+from oracle import cm_eval
+
+
+def ulp_err64(got, ref):
+    return np.abs(got - ref) / np.spacing(np.abs(ref))
+
+
+def ulp_err32(got, x):
+    ref32 = np.float32(x)
+    return np.abs(got - ref32.astype(np.float64)) / np.spacing(np.abs(ref32)).astype(np.float64)
+
+
+def test_exp_double_accuracy():
+    rng = np.random.default_rng(1)
+    x = np.concatenate([rng.uniform(-700, 700, 200000), rng.uniform(-1, 1, 100000), rng.uniform(-1e-6, 1e-6, 1000)])
+    got = cm_eval("exp", x)
+    assert ulp_err64(got, np.exp(x)).max() <= 2.0
+
+
+def test_log_double_accuracy():
+    rng = np.random.default_rng(2)
+    x = np.concatenate([10 ** rng.uniform(-300, 300, 200000), rng.uniform(0.5, 2.0, 100000), 1 + rng.uniform(-1e-8, 1e-8, 1000)])
+    got = cm_eval("log", x)
+    ref = np.log(x)
+    ok = np.abs(ref) > 0
+    assert ulp_err64(got[ok], ref[ok]).max() <= 2.0
+
+
+def test_exp_log_float_accuracy():
+    rng = np.random.default_rng(3)
+    x = rng.uniform(-87, 88, 200000).astype(np.float32).astype(np.float64)
+    assert ulp_err32(cm_eval("exp", x, precision="single"), np.exp(x)).max() <= 2.0
+    y = (10 ** rng.uniform(-37, 38, 200000)).astype(np.float32).astype(np.float64)
+    ref = np.log(y)
+    ok = np.abs(ref) > 1e-30
+    assert ulp_err32(cm_eval("log", y, precision="single")[ok], ref[ok]).max() <= 2.0
+
+
+def test_pow_accuracy():
+    rng = np.random.default_rng(4)
+    x = 10 ** rng.uniform(-5, 5, 100000)
+    y = rng.uniform(-10, 10, 100000)
+    ref = np.power(x, y)
+    ok = np.isfinite(ref) & (ref > 1e-300) & (ref < 1e300)
+    assert ulp_err64(cm_eval("pow", x, y)[ok], ref[ok]).max() <= 2
+    # the controller's use: (rtol/err)^(1/5), (rtol/err)^(1/3)
+    q = 10 ** rng.uniform(-8, 8, 10000)
+    for e in (0.2, 1.0 / 3.0):
+        assert ulp_err64(cm_eval("pow", q, np.full_like(q, e)), np.power(q, e)).max() <= 4
+
+
+def test_special_cases():
+    inf, nan = np.inf, np.nan
+    x = np.array([0.0, -0.0, inf, -inf, nan, 710.0, -746.0])
+    e = cm_eval("exp", x)
+    assert e[0] == 1 and e[1] == 1 and e[2] == inf and e[3] == 0 and np.isnan(e[4]) and e[5] == inf and e[6] == 0
+    lg = cm_eval("log", np.array([0.0, -1.0, inf, 1.0, nan]))
+    assert lg[0] == -inf and np.isnan(lg[1]) and lg[2] == inf and lg[3] == 0 and np.isnan(lg[4])
+    px = np.array([2.0, -2.0, -2.0, 0.0, 0.0, inf, 1.0, nan, -8.0])
+    py = np.array([0.0, 3.0, 0.5, 2.0, -1.0, 0.5, nan, 0.0, 1.0 / 3.0])
+    p = cm_eval("pow", px, py)
+    assert p[0] == 1 and p[1] == -8 and np.isnan(p[2]) and p[3] == 0 and p[4] == inf and p[5] == inf
+    assert p[6] == 1 and p[7] == 1 and np.isnan(p[8])
+
+
+@pytest.mark.parametrize("n", [0, 1, 2, 3, 5, 7, -1, -2, -3])
+def test_pown_exact(n):
+    x = np.array([1.5, -2.25, 0.3, 7.0])
+    got = cm_eval("pown", x, np.full_like(x, n))
+    ref = np.array([float(np.float64(v) ** n) for v in x])
+    assert np.allclose(got, ref, rtol=4e-16, atol=0)

@@ -1,0 +1,147 @@
+"""Pins the oracle's observers with signals whose features are known in
+closed form, and with the reference's own observer tests:
+  * test/test_features.py: x' = cos t from 0 over [0, 4 pi] with thresh2:
+    up-crossings at pi/4 and 9 pi/4, down-crossings at 3 pi/2 and 7 pi/2,
+    event count exactly 2;
+  * test/test_aux_values.py: aux constants 1 and -2 reported exactly, the
+    mean/min/max of x + 1 ~ 1/0/2;
+  * test/test_vdp.py: Van der Pol period against its asymptotic formula;
+  * harmonic oscillator: period 2 pi / omega and amplitude for localmax,
+    thresh2 and nhood2.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import OracleEnsemble
+
+OBS_BASE = dict(e_var_ix=0, f_var_ix=0, max_event_count=100, max_event_timestamps=0, min_amp=0.0, min_imi=0.0,
+                nhood_radius=0.05, x_up_threshold=0.3, x_down_threshold=0.2, dx_up_threshold=0.0,
+                dx_down_threshold=0.0, eps_dx=0.0)
+
+
+def feature_sim(model, nv, npar, naux, stepper, observer, prec, sp, obs=None):
+    op = dict(OBS_BASE)
+    op.update(obs or {})
+    o = OracleEnsemble(model, nv, npar, naux, stepper=stepper, observer=observer, precision=prec,
+                       n_store=op["max_event_timestamps"], threads=2)
+    o.set_solver_params(*sp)
+    o.set_observer_params(**op)
+    return o
+
+
+def names_thresh2(vars_, aux, S):
+    f = [f"{p} {s}" for s in ("period", "peaks", "upDuration", "downDuration", "duty", "activeDip")
+         for p in ("max", "min", "mean")]
+    for v in vars_:
+        f += [f"max {v}", f"min {v}", f"mean {v}", f"max d{v}/dt", f"min d{v}/dt"]
+    for a in aux:
+        f += [f"max {a}", f"min {a}", f"mean {a}"]
+    for j in range(S):
+        f += [f"up event time {j}", f"down event time {j}"]
+    return f + ["event count", "step count", "max dt", "min dt", "mean dt"]
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_thresh2_sine_event_times(prec):
+    o = feature_sim("sine", 1, 1, 3, "rk4", "thresh2", prec, (0.001, 0.001, 1e-6, 1e-3, 10 ** 7, 10 ** 7, 1),
+                    dict(min_amp=0.5, x_up_threshold=(2 + math.sqrt(2)) / 4, x_down_threshold=0.001,
+                         dx_up_threshold=0.001, dx_down_threshold=0.001, max_event_timestamps=3))
+    o.set_problem_data(np.zeros((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, 4 * math.pi))
+    o.features()
+    F = dict(zip(names_thresh2(["x"], ["xp1", "pos", "neg"], 3), o.get("F")))
+    assert F["event count"] == 2
+    assert abs(F["up event time 0"] - math.pi / 4) < 1e-2
+    assert abs(F["up event time 1"] - 9 * math.pi / 4) < 1e-2
+    assert abs(F["down event time 0"] - 3 * math.pi / 2) < 1e-2
+    assert abs(F["down event time 1"] - 7 * math.pi / 2) < 1e-2
+    assert F["up event time 2"] == 0 and F["down event time 2"] == 0
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_thresh2_aux_values(prec):
+    o = feature_sim("sine", 1, 1, 3, "rk4", "thresh2", prec, (0.1, 0.1, 1e-6, 1e-3, 10 ** 7, 10 ** 7, 1))
+    o.set_problem_data(np.zeros((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, 400 * math.pi))
+    o.features()
+    F = dict(zip(names_thresh2(["x"], ["xp1", "pos", "neg"], 0), o.get("F")))
+    assert F["mean xp1"] == pytest.approx(1.0, abs=1e-2)
+    assert F["min xp1"] == pytest.approx(0.0, abs=1e-2)
+    assert F["max xp1"] == pytest.approx(2.0, abs=1e-2)
+    assert F["mean pos"] == 1.0 and F["min pos"] == 1.0 and F["max pos"] == 1.0
+    assert F["min neg"] == -2.0 and F["max neg"] == -2.0
+
+
+def vdp_period(mu):
+    if mu < 0:
+        return 0.0
+    if mu < 2:
+        return 2 * math.pi * (1 + mu ** 2 / 16)
+    return min(2 * math.pi * (1 + mu ** 2 / 16), (3 - 2 * math.log(2)) * mu + 3 * 2.2338 / mu ** (1 / 3))
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_thresh2_van_der_pol_period(prec):
+    mus = np.array([-1, 0, 0.01, 0.1, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0, 5, 6])
+    o = feature_sim("van_der_pol", 2, 1, 0, "dopri5", "thresh2", prec, (0.1, 1.0, 1e-6, 1e-3, 20000, 20000, 1))
+    o.set_problem_data(np.ones((2, mus.size)), mus[None, :])
+    o.set_tspan((0.0, 1000.0))
+    o.features()
+    F = o.get("F").reshape(-1, mus.size)
+    for mu, p in zip(mus, F[0]):  # "max period"
+        assert p == pytest.approx(vdp_period(mu), rel=1e-2, abs=1.0), mu
+
+
+def harmonic_features(observer, omega=2.0, A=1.5, prec="double", obs=None):
+    o = feature_sim("harmonic", 2, 1, 0, "rk4", observer, prec, (0.001, 0.001, 1e-6, 1e-3, 10 ** 7, 10 ** 7, 1), obs)
+    o.set_problem_data(np.array([[A], [0.0]]), np.array([[omega]]))
+    o.set_tspan((0.0, 40.0))
+    o.features()
+    return o.get("F")
+
+
+def test_localmax_harmonic_period_and_amplitude():
+    omega, A = 2.0, 1.5
+    F = harmonic_features("localmax", omega, A)
+    # IMI max/min/mean, amplitude max/min/mean
+    assert F[2] == pytest.approx(2 * math.pi / omega, abs=2e-3)
+    assert F[5] == pytest.approx(2 * A, abs=1e-4)
+    # per variable: max, min, mean, max dx, min dx (x then y)
+    assert F[6] == pytest.approx(A, abs=1e-6) and F[7] == pytest.approx(-A, abs=1e-4)
+    assert F[9] == pytest.approx(omega * A, abs=1e-4)  # max dx/dt = omega*A
+    # event count: maxima of A cos(2 t) at t = pi k, k = 1..12 in (0, 40]
+    assert F[-2] == 12
+
+
+def test_thresh2_harmonic_period():
+    omega = 2.0
+    F = harmonic_features("thresh2", omega)
+    assert F[2] == pytest.approx(2 * math.pi / omega, abs=2e-3)   # mean period
+    assert F[5] == 1.0                                          # one peak per period
+    # up: rising crossing of -0.4 A (phase 2 pi - acos(-0.4)); down: falling crossing of
+    # -0.6 A (phase acos(-0.6)); duty = up-duration / period
+    duty = (math.acos(-0.6) + math.acos(-0.4)) / (2 * math.pi)
+    assert F[14] == pytest.approx(duty, abs=5e-3)  # mean duty
+
+
+def test_nhood2_harmonic_period():
+    omega = 2.0
+    F = harmonic_features("nhood2", omega)
+    assert F[2] == pytest.approx(2 * math.pi / omega, abs=2e-3)
+    # nhood centre: first downward crossing of min + 0.2 range in x
+    nv_block = F[6:6 + 7]
+    assert nv_block[3] == pytest.approx(3.0, abs=1e-5)          # range x = 2A
+    assert nv_block[4] == pytest.approx(-1.5 + 0.2 * 3.0, abs=1e-2)
+
+
+def test_localmax_no_events_for_monotone_decay():
+    o = feature_sim("decay", 1, 1, 0, "rk4", "localmax", "double", (0.01, 0.01, 1e-6, 1e-3, 10 ** 6, 10 ** 6, 1),
+                    dict(max_event_timestamps=2))
+    o.set_problem_data(np.ones((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, 5.0))
+    o.features()
+    F = o.get("F")
+    assert F[-2] == 0 and np.all(F[:6] == 0)
+    assert F[-1] == 501 or F[-1] == 502  # step count (t <= 5 loop)

@@ -1,0 +1,107 @@
+"""Pins the oracle's steppers with closed forms and convergence orders.
+
+A dropped tableau term, a wrong stage time or a transposed coefficient
+changes the observed order or the closed-form error by orders of magnitude,
+which these checks catch:
+  * decay x' = -x and the harmonic oscillator with fixed steps: global error
+    ratios on halving dt give orders 1 (euler), 2 (heun), 4 (rk4);
+  * Euler on decay equals (1 - h)^n to rounding;
+  * adaptive bs23/dopri5: error against the closed form shrinks with rtol
+    (proportional control); the last step ends one hmin past the span end;
+  * the falling body with linear drag (ORNL problem A1, the reference's
+    test/test_ornl_thompson_a1.py) along the stored trajectory to 5
+    significant digits, in single precision as the reference runs it.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from oracle import OracleEnsemble
+
+
+def run_decay(stepper, dt, t_end=1.0, prec="double"):
+    o = OracleEnsemble("decay", 1, 1, stepper=stepper, precision=prec, kind="base", threads=1)
+    o.set_solver_params(dt, dt, 1e-12, 1e-10, 10 ** 8, 1000, 1)
+    o.set_problem_data(np.ones((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, t_end))
+    o.transient()
+    return o.get("xf")[0], o.get("tf")[0]
+
+
+@pytest.mark.parametrize("stepper,order", [("euler", 1), ("heun", 2), ("rk4", 4)])
+def test_fixed_step_orders(stepper, order):
+    # the loop runs while t <= t_end, so the final time overshoots by < dt:
+    # compare against exp(-tf) at the actual final time
+    errs = []
+    for dt in (1 / 64, 1 / 128, 1 / 256):
+        x, tf = run_decay(stepper, dt)
+        errs.append(abs(x - math.exp(-tf)))
+    r1, r2 = errs[0] / errs[1], errs[1] / errs[2]
+    assert abs(math.log2(r1) - order) < 0.15 and abs(math.log2(r2) - order) < 0.15, (errs, r1, r2)
+
+
+def test_euler_is_repeated_multiplication():
+    h = 1 / 32
+    x, tf = run_decay("euler", h, t_end=1.0)
+    n = round(tf / h)
+    assert x == pytest.approx((1 - h) ** n, rel=1e-14)
+
+
+def test_rk4_harmonic_closed_form():
+    o = OracleEnsemble("harmonic", 2, 1, stepper="rk4", kind="base", threads=1)
+    o.set_solver_params(0.01, 0.01, 1e-6, 1e-3, 10 ** 7, 1000, 1)
+    o.set_problem_data(np.array([[1.0], [0.0]]), np.array([[1.0]]))
+    o.set_tspan((0.0, 2 * math.pi))
+    o.transient()
+    x, y = o.get("xf")
+    tf = o.get("tf")[0]
+    assert abs(x - math.cos(tf)) < 1e-9 and abs(y + math.sin(tf)) < 1e-9
+
+
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+def test_adaptive_tolerance_proportionality(stepper):
+    errs = []
+    for rtol in (1e-4, 1e-6, 1e-8):
+        o = OracleEnsemble("decay", 1, 1, stepper=stepper, kind="base", threads=1)
+        o.set_solver_params(0.01, 0.5, rtol * 1e-3, rtol, 10 ** 8, 1000, 1)
+        o.set_problem_data(np.ones((1, 1)), np.ones((1, 1)))
+        o.set_tspan((0.0, 5.0))
+        o.transient()
+        tf = o.get("tf")[0]
+        errs.append(abs(o.get("xf")[0] - math.exp(-tf)) / math.exp(-tf))
+    assert errs[0] > errs[1] > errs[2]
+    assert errs[1] < 1e-4 and errs[2] < 1e-6
+
+
+def test_adaptive_end_of_span_and_dt_carry():
+    # the controller clips the last step to the end of the span and then to
+    # hmin = 16 ulp(t): the loop takes one extra hmin step past t1 and the
+    # stored dt is that hmin (adaptive_explicit_step.clh:68-72, transient.cl:73)
+    o = OracleEnsemble("decay", 1, 1, stepper="dopri5", kind="base", threads=1)
+    o.set_solver_params(0.1, 0.5, 1e-9, 1e-6, 10 ** 6, 1000, 1)
+    o.set_problem_data(np.ones((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, 10.0))
+    o.transient()
+    tf, dt = o.get("tf")[0], o.get("dt")[0]
+    assert 10.0 < tf <= 10.0 + 32 * np.spacing(10.0)
+    assert dt == pytest.approx(16 * np.spacing(tf), rel=0.5)
+
+
+@pytest.mark.parametrize("prec", ["single", "double"])
+def test_linear_drag_trajectory_five_digits(prec):
+    """Reference test/test_ornl_thompson_a1.py: rk4, dt 1e-3, t in [0, 5]."""
+    m, w, k, H = 0.25, 8.0, 2.0, 10.0
+    o = OracleEnsemble("linear_drag", 2, 4, 1, stepper="rk4", precision=prec, kind="trajectory", threads=1)
+    o.set_solver_params(0.001, 0.001, 1e-6, 1e-4, 20000, 20000, 1)
+    o.set_problem_data(np.zeros((2, 1)), np.array([[m], [w], [k], [H]]))
+    o.set_tspan((0.0, H / 2.0))
+    o.trajectory()
+    n = int(o.get("nStored")[0]) + 1
+    t = o.get("t")[:n]
+    x = o.get("x").reshape(-1, 2)[:n]
+    y1 = 4.0 * (t + 1.0 / 8.0 * np.exp(-8.0 * t) - 1.0 / 8.0)
+    y2 = 4.0 * (1.0 - np.exp(-8.0 * t))
+    for a, b in ((x[1:, 0], y1[1:]), (x[1:, 1], y2[1:])):
+        np.testing.assert_allclose(a, b, rtol=5e-5, atol=1e-6 if prec == "single" else 0)
+    assert n == 5001 or n == 5002
