@@ -12,6 +12,7 @@ has not been built (``python -m clode_amd.build``).
 from __future__ import annotations
 
 from .cpp.clode_cpp_wrapper import ObserverParams, ProblemInfo, SolverParams
+from .batch import BatchSimulator
 from .features import FeatureSimulator, Observer, ObserverOutput
 from .runtime import (
     CLDeviceType,
@@ -32,6 +33,7 @@ from .trajectory import TrajectoryOutput, TrajectorySimulator
 __version__ = "0.1.0"
 
 __all__ = [
+    "BatchSimulator",
     "CLDeviceType",
     "CLVendor",
     "DeviceInfo",

@@ -49,13 +49,15 @@ struct ClodeArgs {
     void *tr_dx;         // [maxStore][nVar][nPts]
     void *tr_aux;        // [maxStore][nAux][nPts]
     int *nStored;        // [nPts]
-    // persistent / batched launches
-    unsigned int *work_counter;  // next member to claim (persistent kernel)
     void *tspan_m;               // optional per-member [2][nPts] time spans (double)
-    unsigned int *status;        // optional per-member status word
     unsigned long long *steps;   // optional per-member step counter (accumulated)
-    unsigned int member_offset;  // first member handled by this launch
-    unsigned int n_launch;       // members handled by this launch
+    // work queue (persistent launches): position q -> member q_order[q*q_stride]
+    unsigned int *work_counter;  // next queue position to claim; zeroed before each launch
+    const unsigned int *q_order; // optional queue order (member indices)
+    unsigned int q_count;        // queue positions
+    unsigned int q_stride;       // stride through the order (1 = every member)
+    unsigned int q_slice;        // loop iterations between queue checks
+    unsigned int q_pad;
 };
 
 #ifdef __cplusplus

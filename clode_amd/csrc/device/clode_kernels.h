@@ -1,10 +1,17 @@
 // Ensemble drivers: one member per lane, state in registers for the whole
 // integration, coalesced structure-of-arrays loads/stores at the ends.
 //
-//   clode_transient           clode/cpp/transient.cl:9-77
-//   clode_initialize_observer clode/cpp/initializeObserver.cl:9-84
-//   clode_features            clode/cpp/features.cl:10-106
-//   clode_trajectory          clode/cpp/trajectory.cl:14-112
+//   TransientLane  clode/cpp/transient.cl:9-77
+//   InitObsLane    clode/cpp/initializeObserver.cl:9-84
+//   FeaturesLane   clode/cpp/features.cl:10-106
+//   TrajectoryLane clode/cpp/trajectory.cl:14-112
+//
+// Each driver is a "lane" with begin (load member i), running (the loop
+// condition), advance (one loop iteration) and finish (store member i).  The
+// grid kernels run one member per lane to completion; the work-queue kernels
+// (run_queue) refill a lane with the next unclaimed member as soon as its
+// member finishes, so members of very different lengths keep every lane busy.
+// Both run the same lane code, so their results are identical.
 //
 // Semantics kept from the reference drivers: the time loop runs while
 // t <= tspan[1] (t < tspan[1] in the observer warm-up pass) and
@@ -14,7 +21,13 @@
 // features pass replays the same noise; the trajectory stores the initial
 // point and then every nout-th step.  One deliberate change: the trajectory
 // never writes sample index max_store (the reference can write one sample past
-// the end of its buffers); nStored still counts that last step.
+// the end of its buffers); nStored still counts that last step.  Optional
+// per-member time spans (tspan_m, [2][nPts]) replace the global span member by
+// member; the reference has one span per launch.
+//
+// The grid kernels are not emitted when CLODE_BATCH is defined: a batched
+// program compiles these drivers once per system, each inside its own
+// namespace, and generates one dispatching work-queue kernel.
 #ifndef CLODE_KERNELS_H
 #define CLODE_KERNELS_H
 
@@ -161,6 +174,19 @@ CLODE_DEV void count_steps(const ClodeArgs &a, unsigned int i, unsigned int step
         a.steps[i] += step;
 }
 
+// the member's time span: the launch's, or its own when tspan_m is given
+CLODE_DEV void member_tspan(const ClodeArgs &a, unsigned int i, timetype &t0, timetype &t1)
+{
+    if (a.tspan_m) {
+        const double *tm = (const double *)a.tspan_m;
+        t0 = (timetype)tm[i];
+        t1 = (timetype)tm[(size_t)a.nPts + i];
+    } else {
+        t0 = (timetype)a.t0;
+        t1 = (timetype)a.t1;
+    }
+}
+
 #if CLODE_PREC == 3
 #define CLODE_STEP(m)                                                                              \
     stepper(m.ti, m.xi, m.xc, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd, a.sp_reltol,            \
@@ -169,100 +195,131 @@ CLODE_DEV void count_steps(const ClodeArgs &a, unsigned int i, unsigned int step
 #define CLODE_STEP(m) stepper(m.ti, m.xi, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd)
 #endif
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_transient(const ClodeArgs a)
+// common start of every driver: load the member, draw the initial noise and
+// evaluate the right-hand side at t0
+CLODE_DEV void begin_member(Member &m, const ClodeArgs &a, unsigned int i, timetype t0)
 {
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nPts)
-        return;
-    const SolverParamsR sp = make_solver_params(a);
-    const timetype t1 = (timetype)a.t1;
-    Member m;
-    m.ti = (timetype)a.t0;
-    load_member(m, a, i);
-    initial_noise(m);
-    getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
-    unsigned int step = 0;
-    while (m.ti <= t1 && step < sp.max_steps) {
-        ++step;
-        CLODE_STEP(m);
-    }
-    store_member(m, a, i);
-    count_steps(a, i, step);
-}
-
-#ifdef CLODE_OBSERVER
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_initialize_observer(const ClodeArgs a)
-{
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nPts)
-        return;
-    const SolverParamsR sp = make_solver_params(a);
-    const ObserverParamsR op = make_observer_params(a);
-    const timetype t0 = (timetype)a.t0;
-    const timetype t1 = (timetype)a.t1;
-    ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
-    Member m;
     m.ti = t0;
     load_member(m, a, i);
     initial_noise(m);
     getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
+}
+
+struct TransientLane {
+    Member m;
+    timetype t1;
+    unsigned int i, step;
+    CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
+    {
+        i = idx;
+        timetype t0;
+        member_tspan(a, i, t0, t1);
+        begin_member(m, a, i, t0);
+        step = 0;
+    }
+    CLODE_DEV bool running(const SolverParamsR &sp) const { return m.ti <= t1 && step < sp.max_steps; }
+    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    {
+        ++step;
+        CLODE_STEP(m);
+    }
+    CLODE_DEV void finish(const ClodeArgs &a)
+    {
+        store_member(m, a, i);
+        count_steps(a, i, step);
+    }
+};
+
+#ifdef CLODE_OBSERVER
+#ifndef CLODE_TWO_PASS
+#define CLODE_TWO_PASS 0
+#endif
+
+struct InitObsLane {
+    Member m;
     Observer ob;
-    ob.init(m.ti, m.xi, m.dxi, m.auxi, op, io);
-#if CLODE_TWO_PASS
-    unsigned int step = 0;
-    while (m.ti < t1 && step < sp.max_steps) {
+    ObserverParamsR op;
+    timetype t0, t1;
+    unsigned int i, step;
+    CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
+    {
+        i = idx;
+        op = make_observer_params(a);
+        member_tspan(a, i, t0, t1);
+        begin_member(m, a, i, t0);
+        ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
+        ob.init(m.ti, m.xi, m.dxi, m.auxi, op, io);
+        step = 0;
+    }
+    CLODE_DEV bool running(const SolverParamsR &sp) const
+    {
+        return CLODE_TWO_PASS && m.ti < t1 && step < sp.max_steps;
+    }
+    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    {
         ++step;
         CLODE_STEP(m);
         ob.warmup(m.ti, m.xi, m.dxi, m.auxi, op);
     }
-    m.ti = t0;
-    load_state(m, a, i);
-    getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
-    count_steps(a, i, step);
-#endif
-    ob.initEventDetector(m.ti, m.xi, m.dxi, m.auxi, op);
-    ObsStore st{io};
-    ob.fields(st);
-}
-
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_features(const ClodeArgs a)
-{
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nPts)
-        return;
-    const SolverParamsR sp = make_solver_params(a);
-    const ObserverParamsR op = make_observer_params(a);
-    const timetype t0 = (timetype)a.t0;
-    const timetype t1 = (timetype)a.t1;
-    ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
-    Member m;
-    m.ti = t0;
-    load_member(m, a, i);
-    Observer ob;
+    CLODE_DEV void finish(const ClodeArgs &a)
     {
-        ObsLoad ld{io};
-        ob.fields(ld);
+        if (CLODE_TWO_PASS) {
+            m.ti = t0;
+            load_state(m, a, i);
+            getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
+            count_steps(a, i, step);
+        }
+        ob.initEventDetector(m.ti, m.xi, m.dxi, m.auxi, op);
+        ObsStore st{ObsIO{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0}};
+        ob.fields(st);
     }
-    initial_noise(m);
-    getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
-    unsigned int step = 0;
-    while (m.ti <= t1 && step < sp.max_steps) {
+};
+
+struct FeaturesLane {
+    Member m;
+    Observer ob;
+    ObserverParamsR op;
+    timetype t0, t1;
+    unsigned int i, step;
+    bool terminal;
+    CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
+    {
+        i = idx;
+        op = make_observer_params(a);
+        member_tspan(a, i, t0, t1);
+        m.ti = t0;
+        load_member(m, a, i);
+        {
+            ObsLoad ld{ObsIO{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0}};
+            ob.fields(ld);
+        }
+        initial_noise(m);
+        getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
+        step = 0;
+        terminal = false;
+    }
+    CLODE_DEV bool running(const SolverParamsR &sp) const { return !terminal && m.ti <= t1 && step < sp.max_steps; }
+    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    {
+        const ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
         ++step;
         CLODE_STEP(m);
         ob.update(m.ti, m.xi, m.dxi, m.auxi, op, io);
-        if (ob.eventFunction(m.ti, m.xi, m.dxi, m.auxi, op)) {
-            if (ob.computeEventFeatures(m.ti, m.xi, m.dxi, m.auxi, op, io))
-                break;
-        }
+        if (ob.eventFunction(m.ti, m.xi, m.dxi, m.auxi, op))
+            terminal = ob.computeEventFeatures(m.ti, m.xi, m.dxi, m.auxi, op, io);
     }
-    FeatOut F{(ostore *)a.F, a.nPts, i, 0};
-    ob.finalizeFeatures(m.ti, F, io);
-    ob.finalizeObserverData(m.ti, t0);
-    ObsStore st{io};
-    ob.fields(st);
-    store_member(m, a, i);
-    count_steps(a, i, step);
-}
+    CLODE_DEV void finish(const ClodeArgs &a)
+    {
+        const ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
+        FeatOut F{(ostore *)a.F, a.nPts, i, 0};
+        ob.finalizeFeatures(m.ti, F, io);
+        ob.finalizeObserverData(m.ti, t0);
+        ObsStore st{io};
+        ob.fields(st);
+        store_member(m, a, i);
+        count_steps(a, i, step);
+    }
+};
 #endif  // CLODE_OBSERVER
 
 CLODE_DEV void trajectory_store(const ClodeArgs &a, unsigned int i, unsigned int s, const Member &m)
@@ -282,22 +339,26 @@ CLODE_DEV void trajectory_store(const ClodeArgs &a, unsigned int i, unsigned int
         AUX[((size_t)s * N_AUX + j) * n + i] = m.auxi[j];
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_trajectory(const ClodeArgs a)
-{
-    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= a.nPts)
-        return;
-    const SolverParamsR sp = make_solver_params(a);
-    const timetype t1 = (timetype)a.t1;
+struct TrajectoryLane {
     Member m;
-    m.ti = (timetype)a.t0;
-    load_member(m, a, i);
-    initial_noise(m);
-    getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
-    unsigned int storeix = 0;
-    trajectory_store(a, i, storeix, m);
-    unsigned int step = 0;
-    while (m.ti <= t1 && step < sp.max_steps && storeix < sp.max_store) {
+    timetype t1;
+    unsigned int i, step, storeix;
+    CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
+    {
+        i = idx;
+        timetype t0;
+        member_tspan(a, i, t0, t1);
+        begin_member(m, a, i, t0);
+        storeix = 0;
+        trajectory_store(a, i, storeix, m);
+        step = 0;
+    }
+    CLODE_DEV bool running(const SolverParamsR &sp) const
+    {
+        return m.ti <= t1 && step < sp.max_steps && storeix < sp.max_store;
+    }
+    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    {
         ++step;
         CLODE_STEP(m);
         if (step % sp.nout == 0) {
@@ -306,9 +367,111 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_trajectory(c
                 trajectory_store(a, i, storeix, m);
         }
     }
-    a.nStored[i] = (int)storeix;
-    store_member(m, a, i);
-    count_steps(a, i, step);
+    CLODE_DEV void finish(const ClodeArgs &a)
+    {
+        a.nStored[i] = (int)storeix;
+        store_member(m, a, i);
+        count_steps(a, i, step);
+    }
+};
+
+// one member per lane, to completion
+template <class Lane>
+CLODE_DEV void run_grid(const ClodeArgs &a)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= a.nPts)
+        return;
+    const SolverParamsR sp = make_solver_params(a);
+    Lane L;
+    L.begin(a, i);
+    while (L.running(sp))
+        L.advance(a, sp);
+    L.finish(a);
 }
+
+// Work queue: queue position q maps to member order[q * q_stride] (or
+// q * q_stride without an order array); positions are claimed with one atomic
+// add on work_counter, which the host zeroes before the launch.
+CLODE_DEV bool claim_member(const ClodeArgs &a, unsigned int &idx)
+{
+    const unsigned int q = atomicAdd(a.work_counter, 1u);
+    if (q >= a.q_count)
+        return false;
+    const unsigned int pos = q * a.q_stride;
+    idx = a.q_order ? a.q_order[pos] : pos;
+    return true;
+}
+
+// Persistent lanes: run up to q_slice iterations of the member's loop, then
+// check; a lane whose member has finished stores it and claims the next one.
+// Every iteration advances a member whose step count is bounded by max_steps
+// or claims a queue position, so every lane leaves the loop once the queue is
+// empty.
+template <class Lane>
+CLODE_DEV void run_queue(const ClodeArgs &a)
+{
+    const SolverParamsR sp = make_solver_params(a);
+    const unsigned int slice = a.q_slice > 0 ? a.q_slice : 1u;
+    Lane L;
+    unsigned int idx = 0;
+    bool have = claim_member(a, idx);
+    if (have)
+        L.begin(a, idx);
+    while (have) {
+        for (unsigned int k = 0; k < slice && L.running(sp); ++k)
+            L.advance(a, sp);
+        if (!L.running(sp)) {
+            L.finish(a);
+            have = claim_member(a, idx);
+            if (have)
+                L.begin(a, idx);
+        }
+    }
+}
+
+#ifndef CLODE_BATCH
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_transient(const ClodeArgs a)
+{
+    run_grid<TransientLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_trajectory(const ClodeArgs a)
+{
+    run_grid<TrajectoryLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_transient(const ClodeArgs a)
+{
+    run_queue<TransientLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_trajectory(const ClodeArgs a)
+{
+    run_queue<TrajectoryLane>(a);
+}
+
+#ifdef CLODE_OBSERVER
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_initialize_observer(const ClodeArgs a)
+{
+    run_grid<InitObsLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_features(const ClodeArgs a)
+{
+    run_grid<FeaturesLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_initialize_observer(const ClodeArgs a)
+{
+    run_queue<InitObsLane>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_features(const ClodeArgs a)
+{
+    run_queue<FeaturesLane>(a);
+}
+#endif  // CLODE_OBSERVER
+#endif  // CLODE_BATCH
 
 #endif  // CLODE_KERNELS_H

@@ -8,6 +8,9 @@
 //   thresh2  clode/cpp/observers/observer_threshold_2.clh:121-470
 //   localmax clode/cpp/observers/observer_local_maximum.clh:92-328
 //   nhood2   clode/cpp/observers/observer_neighborhood_2.clh:90-313
+//   basic    clode/cpp/observers/observer_basic.clh:17-90
+//   basicall clode/cpp/observers/observer_basic_allVar.clh:30-124
+//   nhood1   clode/cpp/observers/observer_neighborhood_1.clh:40-310
 //   helpers  clode/cpp/clODE_utilities.cl:31-177 (runningMean,
 //            runningMeanTime, first-occurrence arg-max/arg-min, norm_2)
 //
@@ -964,5 +967,399 @@ struct Observer {
     }
 };
 #endif  // nhood2
+
+// ------------------------------------------------------------------------
+// basic (observer_basic.clh:17-90) and basicall (observer_basic_allVar.clh:30-124):
+// one-pass trajectory extents and time means, no events.  basic tracks the
+// feature variable only.  The time step is measured from the previous update
+// (t_last), which the continuation shift leaves unchanged, as in the reference.
+#if CLODE_OBSERVER == CLODE_OBS_BASIC || CLODE_OBSERVER == CLODE_OBS_BASICALL
+#if CLODE_OBSERVER == CLODE_OBS_BASIC
+#define CLODE_BASIC_NV 1
+#define CLODE_BASIC_NA 0
+#define CLODE_BASIC_VAR(j) CLODE_FVAR
+#else
+#define CLODE_BASIC_NV N_VAR
+#define CLODE_BASIC_NA N_AUX
+#define CLODE_BASIC_VAR(j) (j)
+#endif
+struct Observer {
+    static constexpr int NSCALAR = clode_obs_nscalar(CLODE_OBSERVER, N_VAR, N_AUX);
+    realtype xMax[CLODE_BASIC_NV], xMin[CLODE_BASIC_NV], dxMax[CLODE_BASIC_NV], dxMin[CLODE_BASIC_NV];
+    obsreal xMean[CLODE_BASIC_NV];  // running time mean (plain) or integral (fp32c)
+    realtype auxMax[N_AUX1], auxMin[N_AUX1];
+    obsreal auxMean[N_AUX1];
+    timetype t_last, t_start;
+    unsigned int stepcount;
+
+    template <class F>
+    CLODE_DEV void fields(F &f)
+    {
+#pragma unroll
+        for (int j = 0; j < CLODE_BASIC_NV; j++) {
+            f.real(xMax[j]); f.real(xMin[j]); f.real(xMean[j]); f.real(dxMax[j]); f.real(dxMin[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < CLODE_BASIC_NA; j++) {
+            f.real(auxMax[j]); f.real(auxMin[j]); f.real(auxMean[j]);
+        }
+        f.real(t_last); f.real(t_start);
+        f.uint(stepcount);
+    }
+
+    CLODE_DEV void init(timetype ti, const realtype xi[], const realtype dxi[], const realtype auxi[],
+                        const ObserverParamsR &op, const ObsIO &io)
+    {
+        for (int j = 0; j < CLODE_BASIC_NV; ++j) {
+            xMax[j] = -BIG_REAL;
+            xMin[j] = BIG_REAL;
+            xMean[j] = ZERO;
+            dxMax[j] = -BIG_REAL;
+            dxMin[j] = BIG_REAL;
+        }
+        for (int j = 0; j < CLODE_BASIC_NA; ++j) {
+            auxMax[j] = -BIG_REAL;
+            auxMin[j] = BIG_REAL;
+            auxMean[j] = ZERO;
+        }
+        t_last = ti;
+        t_start = ti;
+        stepcount = 0;
+    }
+
+    CLODE_DEV void warmup(timetype, const realtype[], const realtype[], const realtype[], const ObserverParamsR &) {}
+    CLODE_DEV void initEventDetector(timetype, const realtype[], const realtype[], const realtype[],
+                                     const ObserverParamsR &) {}
+    CLODE_DEV bool eventFunction(timetype, const realtype[], const realtype[], const realtype[],
+                                 const ObserverParamsR &)
+    {
+        return false;
+    }
+    CLODE_DEV bool computeEventFeatures(timetype, const realtype[], const realtype[], const realtype[],
+                                        const ObserverParamsR &, const ObsIO &)
+    {
+        return false;
+    }
+
+    CLODE_DEV void update(timetype ti, const realtype xi[], const realtype dxi[], const realtype auxi[],
+                          const ObserverParamsR &op, const ObsIO &io)
+    {
+        ++stepcount;
+#if CLODE_PREC == 3
+        double dt = ti - t_last;
+        t_last = ti;
+        for (int j = 0; j < CLODE_BASIC_NV; ++j) {
+            const int v = CLODE_BASIC_VAR(j);
+            xMax[j] = fmaxf(xi[v], xMax[j]);
+            xMin[j] = fminf(xi[v], xMin[j]);
+            xMean[j] = fma((double)xi[v], dt, xMean[j]);
+            dxMax[j] = fmaxf(dxi[v], dxMax[j]);
+            dxMin[j] = fminf(dxi[v], dxMin[j]);
+        }
+        for (int j = 0; j < CLODE_BASIC_NA; ++j) {
+            auxMax[j] = fmaxf(auxi[j], auxMax[j]);
+            auxMin[j] = fminf(auxi[j], auxMin[j]);
+            auxMean[j] = fma((double)auxi[j], dt, auxMean[j]);
+        }
+#else
+        realtype dt = ti - t_last;
+        t_last = ti;
+        realtype elapsedTime = ti - t_start;
+        for (int j = 0; j < CLODE_BASIC_NV; ++j) {
+            const int v = CLODE_BASIC_VAR(j);
+            xMax[j] = fmax(xi[v], xMax[j]);
+            xMin[j] = fmin(xi[v], xMin[j]);
+            xMean[j] = runningMeanTime(xMean[j], xi[v], dt, elapsedTime);
+            dxMax[j] = fmax(dxi[v], dxMax[j]);
+            dxMin[j] = fmin(dxi[v], dxMin[j]);
+        }
+        for (int j = 0; j < CLODE_BASIC_NA; ++j) {
+            auxMax[j] = fmax(auxi[j], auxMax[j]);
+            auxMin[j] = fmin(auxi[j], auxMin[j]);
+            auxMean[j] = runningMeanTime(auxMean[j], auxi[j], dt, elapsedTime);
+        }
+#endif
+    }
+
+    CLODE_DEV void finalizeFeatures(timetype ti, FeatOut &F, const ObsIO &io)
+    {
+#if CLODE_PREC == 3
+        double elapsed = ti - t_start;
+#endif
+        for (int j = 0; j < CLODE_BASIC_NV; ++j) {
+            F.put(xMax[j]);
+            F.put(xMin[j]);
+#if CLODE_PREC == 3
+            F.put(elapsed != 0.0 ? xMean[j] / elapsed : 0.0);
+#else
+            F.put(xMean[j]);
+#endif
+            F.put(dxMax[j]);
+            F.put(dxMin[j]);
+        }
+        for (int j = 0; j < CLODE_BASIC_NA; ++j) {
+            F.put(auxMax[j]);
+            F.put(auxMin[j]);
+#if CLODE_PREC == 3
+            F.put(elapsed != 0.0 ? auxMean[j] / elapsed : 0.0);
+#else
+            F.put(auxMean[j]);
+#endif
+        }
+        F.put((obsreal)stepcount);
+    }
+
+    CLODE_DEV void finalizeObserverData(timetype ti, timetype t0)
+    {
+        timetype T = ti - t0;
+        t_start -= T;
+    }
+};
+#endif  // basic, basicall
+
+// ------------------------------------------------------------------------
+// nhood1 (observer_neighborhood_1.clh:40-310): one-pass neighbourhood return.
+// The centre is the state at the first local minimum of the event variable;
+// an event is the entry into the L2 ball of radius nHoodRadius in
+// range-normalised coordinates (ranges from the running extents).  Means are
+// step-count means.  Of the 3-point buffers, x is kept for every variable (the
+// centre takes all components), dx only for the event and feature variables.
+#if CLODE_OBSERVER == CLODE_OBS_NHOOD1
+struct Observer {
+    static constexpr int NSCALAR = clode_obs_nscalar(CLODE_OBS_NHOOD1, N_VAR, N_AUX);
+    timetype tb0, tb1, tb2;
+    realtype xb0[N_VAR], xb1[N_VAR], xb2[N_VAR];
+    realtype dxe1, dxe2, dxf1, dxf2;
+    realtype x0[N_VAR], xMax[N_VAR], xMin[N_VAR], dxMax[N_VAR], dxMin[N_VAR];
+    obsreal xMean[N_VAR];  // running mean (plain) or sum (fp32c)
+    realtype auxMax[N_AUX1], auxMin[N_AUX1];
+    obsreal auxMean[N_AUX1];
+    obsreal nMaxima[3], period[3], stepDt[3];
+    timetype t_start, tLastEvent;
+    unsigned int thisNMaxima, eventcount, stepcount, foundX0, isInNhood;
+
+    template <class F>
+    CLODE_DEV void fields(F &f)
+    {
+        f.real(tb0); f.real(tb1); f.real(tb2);
+#pragma unroll
+        for (int j = 0; j < N_VAR; j++) {
+            f.real(xb0[j]); f.real(xb1[j]); f.real(xb2[j]);
+        }
+        f.real(dxe1); f.real(dxe2); f.real(dxf1); f.real(dxf2);
+#pragma unroll
+        for (int j = 0; j < N_VAR; j++) {
+            f.real(x0[j]); f.real(xMax[j]); f.real(xMin[j]); f.real(xMean[j]); f.real(dxMax[j]); f.real(dxMin[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < N_AUX; j++) {
+            f.real(auxMax[j]); f.real(auxMin[j]); f.real(auxMean[j]);
+        }
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            f.real(nMaxima[j]); f.real(period[j]); f.real(stepDt[j]);
+        }
+        f.real(t_start); f.real(tLastEvent);
+        f.uint(thisNMaxima); f.uint(eventcount); f.uint(stepcount); f.uint(foundX0); f.uint(isInNhood);
+    }
+
+    CLODE_DEV void init(timetype ti, const realtype xi[], const realtype dxi[], const realtype auxi[],
+                        const ObserverParamsR &op, const ObsIO &io)
+    {
+        tb0 = tb1 = tb2 = ti;
+        for (int j = 0; j < N_VAR; ++j) {
+            xb0[j] = xb1[j] = xb2[j] = xi[j];
+            x0[j] = xi[j];
+            xMax[j] = -BIG_REAL;
+            xMin[j] = BIG_REAL;
+            xMean[j] = ZERO;
+            dxMax[j] = -BIG_REAL;
+            dxMin[j] = BIG_REAL;
+        }
+        dxe1 = dxe2 = dxi[CLODE_EVAR];
+        dxf1 = dxf2 = dxi[CLODE_FVAR];
+        for (int j = 0; j < N_AUX; ++j) {
+            auxMax[j] = -BIG_REAL;
+            auxMin[j] = BIG_REAL;
+            auxMean[j] = ZERO;
+        }
+        nMaxima[0] = -BIG_REAL; nMaxima[1] = BIG_REAL; nMaxima[2] = ZERO;
+        period[0] = -BIG_REAL; period[1] = BIG_REAL; period[2] = ZERO;
+        stepDt[0] = -BIG_REAL; stepDt[1] = BIG_REAL; stepDt[2] = ZERO;
+        t_start = ti;
+        tLastEvent = ZERO;
+        thisNMaxima = 0;
+        eventcount = 0;
+        stepcount = 0;
+        foundX0 = 0;
+        isInNhood = 0;
+    }
+
+    CLODE_DEV void warmup(timetype, const realtype[], const realtype[], const realtype[], const ObserverParamsR &) {}
+    CLODE_DEV void initEventDetector(timetype, const realtype[], const realtype[], const realtype[],
+                                     const ObserverParamsR &) {}
+
+    CLODE_DEV bool eventFunction(timetype ti, const realtype xi[], const realtype dxi[], const realtype auxi[],
+                                 const ObserverParamsR &op)
+    {
+        if (stepcount < 2 || !foundX0)
+            return false;
+        if (xMax[CLODE_FVAR] - xMin[CLODE_FVAR] < op.minXamp)
+            return false;
+        unsigned int lastInNhood = isInNhood;
+        realtype thisXdiff[N_VAR];
+        for (int j = 0; j < N_VAR; ++j)
+            thisXdiff[j] = fabs(xi[j] - x0[j]) / (xMax[j] - xMin[j]);
+        realtype result = ZERO;
+        for (int k = 0; k < N_VAR; k++)
+            result += thisXdiff[k] * thisXdiff[k];
+        isInNhood = sqrt(result) <= op.nHoodRadius;
+        return isInNhood && !lastInNhood;
+    }
+
+    CLODE_DEV bool computeEventFeatures(timetype ti, const realtype xi[], const realtype dxi[],
+                                        const realtype auxi[], const ObserverParamsR &op, const ObsIO &io)
+    {
+        ++eventcount;
+        timetype tThisEvent = ti;
+        if (eventcount > 1) {
+            nMaxima[0] = fmax((obsreal)thisNMaxima, nMaxima[0]);
+            nMaxima[1] = fmin((obsreal)thisNMaxima, nMaxima[1]);
+            runningMean(nMaxima[2], (obsreal)thisNMaxima, eventcount - 1);
+            obsreal thisPeriod = tThisEvent - tLastEvent;
+            period[0] = fmax(thisPeriod, period[0]);
+            period[1] = fmin(thisPeriod, period[1]);
+            runningMean(period[2], thisPeriod, eventcount - 1);
+        }
+        tLastEvent = tThisEvent;
+        thisNMaxima = 0;
+        return eventcount >= op.maxEventCount;
+    }
+
+    CLODE_DEV void update(timetype ti, const realtype xi[], const realtype dxi[], const realtype auxi[],
+                          const ObserverParamsR &op, const ObsIO &io)
+    {
+        ++stepcount;
+        tb0 = tb1;
+        tb1 = tb2;
+        tb2 = ti;
+        for (int j = 0; j < N_VAR; ++j) {
+            xb0[j] = xb1[j];
+            xb1[j] = xb2[j];
+            xb2[j] = xi[j];
+        }
+        dxe1 = dxe2;
+        dxe2 = dxi[CLODE_EVAR];
+        dxf1 = dxf2;
+        dxf2 = dxi[CLODE_FVAR];
+#if CLODE_PREC == 3
+        double thisDt = tb2 - tb1;
+        stepDt[0] = fmax(thisDt, stepDt[0]);
+        stepDt[1] = fmin(thisDt, stepDt[1]);
+        stepDt[2] += thisDt;
+        for (int j = 0; j < N_VAR; ++j) {
+            xMax[j] = fmaxf(xi[j], xMax[j]);
+            xMin[j] = fminf(xi[j], xMin[j]);
+            xMean[j] += (double)xi[j];
+            dxMax[j] = fmaxf(dxi[j], dxMax[j]);
+            dxMin[j] = fminf(dxi[j], dxMin[j]);
+        }
+        for (int j = 0; j < N_AUX; ++j) {
+            auxMax[j] = fmaxf(auxi[j], auxMax[j]);
+            auxMin[j] = fminf(auxi[j], auxMin[j]);
+            auxMean[j] += (double)auxi[j];
+        }
+#else
+        realtype thisDt = tb2 - tb1;
+        stepDt[0] = fmax(thisDt, stepDt[0]);
+        stepDt[1] = fmin(thisDt, stepDt[1]);
+        runningMean(stepDt[2], thisDt, stepcount);
+        for (int j = 0; j < N_VAR; ++j) {
+            xMax[j] = fmax(xi[j], xMax[j]);
+            xMin[j] = fmin(xi[j], xMin[j]);
+            runningMean(xMean[j], xi[j], stepcount);
+            dxMax[j] = fmax(dxi[j], dxMax[j]);
+            dxMin[j] = fmin(dxi[j], dxMin[j]);
+        }
+        for (int j = 0; j < N_AUX; ++j) {
+            auxMax[j] = fmax(auxi[j], auxMax[j]);
+            auxMin[j] = fmin(auxi[j], auxMin[j]);
+            runningMean(auxMean[j], auxi[j], stepcount);
+        }
+#endif
+        if (stepcount < 2)
+            return;
+        if (!foundX0) {
+            if (dxe1 <= 0.0 && dxe2 > 0.0) {
+                // first-occurrence minimum of the event variable's buffer
+                realtype xm = BIG_REAL;
+                int ix = 0;
+                if (xb0[CLODE_EVAR] < xm) { xm = xb0[CLODE_EVAR]; ix = 0; }
+                if (xb1[CLODE_EVAR] < xm) { xm = xb1[CLODE_EVAR]; ix = 1; }
+                if (xb2[CLODE_EVAR] < xm) { xm = xb2[CLODE_EVAR]; ix = 2; }
+                tLastEvent = ix == 0 ? tb0 : (ix == 1 ? tb1 : tb2);
+                foundX0 = 1;
+                for (int j = 0; j < N_VAR; ++j)
+                    x0[j] = ix == 0 ? xb0[j] : (ix == 1 ? xb1[j] : xb2[j]);
+            }
+        } else if (dxf1 >= 0.0 && dxf2 < 0.0) {
+            thisNMaxima++;
+        }
+    }
+
+    CLODE_DEV void finalizeFeatures(timetype ti, FeatOut &F, const ObsIO &io)
+    {
+        bool p = eventcount > 1;
+        F.put(p ? period[0] : ZERO);
+        F.put(p ? period[1] : ZERO);
+        F.put(p ? period[2] : ZERO);
+        F.put(p ? nMaxima[0] : ZERO);
+        F.put(p ? nMaxima[1] : ZERO);
+        F.put(p ? nMaxima[2] : ZERO);
+#if CLODE_PREC == 3
+        const double inv = stepcount > 0 ? 1.0 / (double)stepcount : 0.0;
+#endif
+        for (int j = 0; j < N_VAR; ++j) {
+            F.put(xMax[j]);
+            F.put(xMin[j]);
+#if CLODE_PREC == 3
+            F.put(xMean[j] * inv);
+#else
+            F.put(xMean[j]);
+#endif
+            F.put(dxMax[j]);
+            F.put(dxMin[j]);
+        }
+        for (int j = 0; j < N_AUX; ++j) {
+            F.put(auxMax[j]);
+            F.put(auxMin[j]);
+#if CLODE_PREC == 3
+            F.put(auxMean[j] * inv);
+#else
+            F.put(auxMean[j]);
+#endif
+        }
+        F.put((obsreal)(unsigned int)(eventcount - 1u));  // periods; wraps at eventcount 0 as in the reference
+        F.put((obsreal)stepcount);
+        F.put(stepDt[0]);
+        F.put(stepDt[1]);
+#if CLODE_PREC == 3
+        F.put(stepDt[2] * inv);
+#else
+        F.put(stepDt[2]);
+#endif
+    }
+
+    CLODE_DEV void finalizeObserverData(timetype ti, timetype t0)
+    {
+        timetype T = ti - t0;
+        t_start -= T;
+        tLastEvent -= T;
+        tb0 -= T;
+        tb1 -= T;
+        tb2 -= T;
+    }
+};
+#endif  // nhood1
 
 #endif  // CLODE_OBSERVERS_H

@@ -349,6 +349,8 @@ public:
     void print_status() { check(clode_print_status(sim_)); }
     // extensions
     void set_launch(int mode, int block) { check(clode_set_launch(sim_, mode, block)); }
+    void set_queue_slice(int slice) { check(clode_set_queue_slice(sim_, slice)); }
+    clode_sim *handle() const { return sim_; }
     void set_member_tspan(const std::vector<double> &t)
     {
         check(clode_set_member_tspan(sim_, t.empty() ? nullptr : t.data(), t.size()));
@@ -506,6 +508,78 @@ std::string vec_str(const std::vector<std::string> &v)
 }
 
 }  // namespace
+
+// Several simulators in one work-queue launch (no reference counterpart); the
+// batch keeps references to its simulators so they outlive it.
+class BatchSimulatorBasePy {
+public:
+    explicit BatchSimulatorBasePy(const py::list &sims)
+    {
+        std::vector<clode_sim *> h;
+        for (const auto &o : sims) {
+            SimulatorBasePy &s = o.cast<SimulatorBasePy &>();
+            h.push_back(s.handle());
+            keep_.push_back(py::reinterpret_borrow<py::object>(o));
+        }
+        check(clode_batch_create(&b_, h.data(), h.size()));
+    }
+    ~BatchSimulatorBasePy() { clode_batch_destroy(b_); }
+    BatchSimulatorBasePy(const BatchSimulatorBasePy &) = delete;
+    BatchSimulatorBasePy &operator=(const BatchSimulatorBasePy &) = delete;
+
+    void set_launch(int block, int slice) { check(clode_batch_set_launch(b_, block, slice)); }
+    py::tuple get_launch()
+    {
+        int b = 0, s = 0;
+        check(clode_batch_get_launch(b_, &b, &s));
+        return py::make_tuple(b, s);
+    }
+    void transient()
+    {
+        py::gil_scoped_release nogil;
+        check(clode_batch_transient(b_));
+    }
+    void initialize_observer()
+    {
+        py::gil_scoped_release nogil;
+        check(clode_batch_initialize_observer(b_));
+    }
+    void features(int reinit)
+    {
+        py::gil_scoped_release nogil;
+        check(clode_batch_features(b_, reinit));
+    }
+    void shift_x0()
+    {
+        py::gil_scoped_release nogil;
+        check(clode_batch_shift_x0(b_));
+    }
+    void sweep(int rounds, bool graph)
+    {
+        py::gil_scoped_release nogil;
+        check(clode_batch_sweep(b_, rounds, graph ? 1 : 0));
+    }
+    py::tuple autotune(int stride)
+    {
+        int b = 0, s = 0;
+        {
+            py::gil_scoped_release nogil;
+            check(clode_batch_autotune(b_, stride, &b, &s));
+        }
+        return py::make_tuple(b, s);
+    }
+    double last_kernel_ms()
+    {
+        double ms = 0;
+        check(clode_batch_last_ms(b_, &ms));
+        return ms;
+    }
+    size_t size() const { return keep_.size(); }
+
+private:
+    clode_batch *b_ = nullptr;
+    std::vector<py::object> keep_;
+};
 
 PYBIND11_MODULE(clode_cpp_wrapper, m)
 {
@@ -713,6 +787,7 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("get_n_pts", &SimulatorBasePy::n_pts)
         .def("get_precision", &SimulatorBasePy::get_precision)
         .def("set_launch", &SimulatorBasePy::set_launch, py::arg("mode"), py::arg("block_size") = 0)
+        .def("set_queue_slice", &SimulatorBasePy::set_queue_slice)
         .def("set_member_tspan", &SimulatorBasePy::set_member_tspan)
         .def("set_dt_refresh", &SimulatorBasePy::set_dt_refresh)
         .def("last_kernel_ms", &SimulatorBasePy::last_kernel_ms)
@@ -751,4 +826,27 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("get_dx", &TrajectorySimulatorBasePy::get_dx)
         .def("get_aux", &TrajectorySimulatorBasePy::get_aux)
         .def("get_n_stored", &TrajectorySimulatorBasePy::get_n_stored);
+
+    m.def(
+        "compile_batch",
+        [](const py::list &sims) {
+            std::vector<clode_sim *> h;
+            for (const auto &o : sims)
+                h.push_back(o.cast<SimulatorBasePy &>().handle());
+            check(clode_batch_compile(h.data(), h.size()));
+        },
+        "Compile the batched program for these simulators (no device needed)");
+
+    py::class_<BatchSimulatorBasePy>(m, "BatchSimulatorBase")
+        .def(py::init<const py::list &>(), py::arg("simulators"))
+        .def("set_launch", &BatchSimulatorBasePy::set_launch, py::arg("block_size"), py::arg("slice"))
+        .def("get_launch", &BatchSimulatorBasePy::get_launch)
+        .def("transient", &BatchSimulatorBasePy::transient)
+        .def("initialize_observer", &BatchSimulatorBasePy::initialize_observer)
+        .def("features", &BatchSimulatorBasePy::features, py::arg("reinitialize") = -1)
+        .def("shift_x0", &BatchSimulatorBasePy::shift_x0)
+        .def("sweep", &BatchSimulatorBasePy::sweep, py::arg("rounds"), py::arg("graph") = true)
+        .def("autotune", &BatchSimulatorBasePy::autotune, py::arg("stride") = 16)
+        .def("last_kernel_ms", &BatchSimulatorBasePy::last_kernel_ms)
+        .def("__len__", &BatchSimulatorBasePy::size);
 }

@@ -1,0 +1,76 @@
+// Batched ensembles: several simulators, possibly with different right-hand
+// sides, dimensions, steppers and observers, integrated by one persistent
+// work-queue launch.
+//
+// The reference runs one system per program and one time span per launch
+// (clode/cpp/CLODE.cpp:148, transient.cl:10); a batch has no counterpart
+// there.  Each member simulator keeps its own buffers, parameters, noise
+// stream and host semantics; the batch compiles all of their systems into one
+// code object (one namespace per system) and launches dispatching kernels
+// whose lanes claim members from per-system work queues.  Per-member time
+// spans (Simulator::setMemberTspan) are honoured, with the queue ordered
+// longest span first.  A member's results are bit-identical to running its
+// simulator on its own, because the same lane code integrates it.
+#pragma once
+
+#include "simulator.h"
+
+namespace clode {
+
+class Batch {
+public:
+    explicit Batch(const std::vector<Simulator *> &sims);
+    ~Batch();
+
+    // workgroup size (64..256, multiple of 64) and loop iterations between
+    // queue checks (>= 1)
+    void setLaunch(int block, int slice);
+    int block() const { return block_; }
+    int slice() const { return slice_; }
+
+    void transient();
+    void initializeObserver();
+    // reinit: -1 continue (initialise where needed), 0 continue, 1 re-initialise
+    // (the semantics of features() / features(false) / features(true))
+    void features(int reinit);
+    void shiftX0();
+
+    // rounds x (transient, x0 <- xf, features, x0 <- xf); with graph = true the
+    // whole sequence is captured into one graph and launched once
+    void sweep(int rounds, bool graph);
+
+    // time the transient work-queue kernel on every stride-th queued member
+    // (state restored afterwards) for each candidate launch shape and keep the
+    // fastest; returns the chosen shape
+    void autotune(int stride, int &block, int &slice);
+
+    // compile the batched program for these simulators (no device needed)
+    static void compileOnly(const std::vector<Simulator *> &sims);
+
+    double lastMs = 0.0;   // device time of the last operation
+    size_t size() const { return sims_.size(); }
+
+private:
+    std::vector<Simulator *> sims_;
+    int device_ = 0, prec_ = 0;
+    bool features_ = false;
+    int block_ = 64, slice_ = 8;
+    std::shared_ptr<Program> prog_;
+    std::string progKey_;
+    hipStream_t stream_ = nullptr;
+    hipEvent_t ev0_ = nullptr, ev1_ = nullptr;
+    DeviceBuffer counters_;
+    std::map<std::pair<void *, int>, size_t> resident_;
+
+    std::shared_ptr<Program> program();
+    unsigned int gridFor(hipFunction_t fn, size_t members);
+    void prepare();
+    std::vector<ClodeArgs> args();
+    void enqueue(const char *kernel, std::vector<ClodeArgs> a);
+    void enqueueShift(std::vector<ClodeArgs> a);
+    void enqueueFeatures(int reinit, std::vector<bool> &initialised);
+    void begin();
+    void end();
+};
+
+}  // namespace clode

@@ -6,12 +6,17 @@
 
 #include <cstring>
 
+#include "batch.h"
 #include "simulator.h"
 
 using namespace clode;
 
 struct clode_sim {
     Simulator *s;
+};
+
+struct clode_batch {
+    Batch *b;
 };
 
 static thread_local std::string g_last_error;
@@ -44,6 +49,12 @@ static thread_local std::string g_last_error;
 #define CHECK_SIM(sim)                                        \
     if (!(sim) || !(sim)->s) {                                \
         g_last_error = "null simulator handle";               \
+        return CLODE_ERR_INVALID;                             \
+    }
+
+#define CHECK_BATCH(batch)                                    \
+    if (!(batch) || !(batch)->b) {                            \
+        g_last_error = "null batch handle";                   \
         return CLODE_ERR_INVALID;                             \
     }
 
@@ -528,6 +539,121 @@ int clode_set_log_level(int level)
 int clode_get_log_level(int *level)
 {
     CLODE_TRY *level = get_log_level();
+    CLODE_CATCH
+}
+
+int clode_set_queue_slice(clode_sim *sim, int slice)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    if (slice < 1)
+        throw InvalidArgument("queue slice must be at least 1");
+    sim->s->queueSlice = (unsigned int)slice;
+    CLODE_CATCH
+}
+
+int clode_batch_create(clode_batch **out, clode_sim *const *sims, size_t n)
+{
+    if (!out) {
+        g_last_error = "null output handle";
+        return CLODE_ERR_INVALID;
+    }
+    *out = nullptr;
+    CLODE_TRY
+    if (!sims)
+        throw InvalidArgument("null simulator array");
+    std::vector<Simulator *> v;
+    for (size_t k = 0; k < n; ++k) {
+        if (!sims[k] || !sims[k]->s)
+            throw InvalidArgument("null simulator handle in batch");
+        v.push_back(sims[k]->s);
+    }
+    auto *h = new clode_batch{nullptr};
+    try {
+        h->b = new Batch(v);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    *out = h;
+    CLODE_CATCH
+}
+int clode_batch_compile(clode_sim *const *sims, size_t n)
+{
+    CLODE_TRY
+    if (!sims)
+        throw InvalidArgument("null simulator array");
+    std::vector<Simulator *> v;
+    for (size_t k = 0; k < n; ++k) {
+        if (!sims[k] || !sims[k]->s)
+            throw InvalidArgument("null simulator handle in batch");
+        v.push_back(sims[k]->s);
+    }
+    Batch::compileOnly(v);
+    CLODE_CATCH
+}
+int clode_batch_destroy(clode_batch *batch)
+{
+    if (!batch)
+        return CLODE_OK;
+    CLODE_TRY
+    delete batch->b;
+    delete batch;
+    CLODE_CATCH
+}
+int clode_batch_set_launch(clode_batch *batch, int block_size, int slice)
+{
+    CHECK_BATCH(batch) CLODE_TRY batch->b->setLaunch(block_size, slice);
+    CLODE_CATCH
+}
+int clode_batch_get_launch(clode_batch *batch, int *block_size, int *slice)
+{
+    CHECK_BATCH(batch) CLODE_TRY
+    *block_size = batch->b->block();
+    *slice = batch->b->slice();
+    CLODE_CATCH
+}
+int clode_batch_transient(clode_batch *batch)
+{
+    CHECK_BATCH(batch) CLODE_TRY batch->b->transient();
+    CLODE_CATCH
+}
+int clode_batch_initialize_observer(clode_batch *batch)
+{
+    CHECK_BATCH(batch) CLODE_TRY batch->b->initializeObserver();
+    CLODE_CATCH
+}
+int clode_batch_features(clode_batch *batch, int reinitialize)
+{
+    CHECK_BATCH(batch) CLODE_TRY
+    if (reinitialize < -1 || reinitialize > 1)
+        throw InvalidArgument("reinitialize must be -1, 0 or 1");
+    batch->b->features(reinitialize);
+    CLODE_CATCH
+}
+int clode_batch_shift_x0(clode_batch *batch)
+{
+    CHECK_BATCH(batch) CLODE_TRY batch->b->shiftX0();
+    CLODE_CATCH
+}
+int clode_batch_sweep(clode_batch *batch, int rounds, int use_graph)
+{
+    CHECK_BATCH(batch) CLODE_TRY batch->b->sweep(rounds, use_graph != 0);
+    CLODE_CATCH
+}
+int clode_batch_autotune(clode_batch *batch, int stride, int *block_size, int *slice)
+{
+    CHECK_BATCH(batch) CLODE_TRY
+    int b = 0, s = 0;
+    batch->b->autotune(stride, b, s);
+    if (block_size)
+        *block_size = b;
+    if (slice)
+        *slice = s;
+    CLODE_CATCH
+}
+int clode_batch_last_ms(clode_batch *batch, double *ms)
+{
+    CHECK_BATCH(batch) CLODE_TRY *ms = batch->b->lastMs;
     CLODE_CATCH
 }
 

@@ -61,7 +61,7 @@ enum Precision { PREC_SINGLE = 1, PREC_DOUBLE = 2, PREC_FP32C = 3 };
 // registries (clode/cpp/steppers.cl:26-43, clode/cpp/observers.cl:122-143)
 const std::map<std::string, int> &stepper_map();
 const std::map<std::string, int> &observer_map();
-std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, int nStore);
+std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, int nStore, unsigned int fVar);
 
 // ------------------------------------------------------------------- device
 struct DeviceBuffer {
@@ -88,7 +88,6 @@ struct ProgramConfig {
     int observer = 0;  // 0 = none
     int eVar = 0, fVar = 0;  // observer event / feature variable
     int nVar = 0, nPar = 0, nAux = 0, nWiener = 0, nStore = 0;
-    int persistent = 0;  // compile the persistent work-queue kernels
     std::string rhs;
     std::string rhs_name;
 };
@@ -109,5 +108,11 @@ private:
 std::string compose_source(const ProgramConfig &cfg);
 std::vector<std::string> compile_options(const ProgramConfig &cfg);
 std::shared_ptr<Program> get_program(const ProgramConfig &cfg, const std::string &cache_dir);
+std::shared_ptr<Program> get_program_source(const std::string &source, const std::vector<std::string> &options,
+                                            const std::string &cache_dir);
+
+// batched programs: up to CLODE_BATCH_MAX systems in one code object
+constexpr size_t CLODE_BATCH_MAX = 8;
+std::string compose_batch_source(const std::vector<ProgramConfig> &cfgs);
 
 }  // namespace clode

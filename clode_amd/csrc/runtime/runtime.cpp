@@ -59,14 +59,16 @@ const std::map<std::string, int> &stepper_map()
 const std::map<std::string, int> &observer_map()
 {
     static const std::map<std::string, int> m = {
-        {"thresh2", CLODE_OBS_THRESH2}, {"localmax", CLODE_OBS_LOCALMAX}, {"nhood2", CLODE_OBS_NHOOD2}};
+        {"thresh2", CLODE_OBS_THRESH2}, {"localmax", CLODE_OBS_LOCALMAX}, {"nhood2", CLODE_OBS_NHOOD2},
+        {"basic", CLODE_OBS_BASIC},     {"basicall", CLODE_OBS_BASICALL}, {"nhood1", CLODE_OBS_NHOOD1}};
     return m;
 }
 
 // feature names, in the order the device writes them
 // (observer_threshold_2.clh:15-53, observer_local_maximum.clh:19-45,
-//  observer_neighborhood_2.clh:14-41)
-std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, int nStore)
+//  observer_neighborhood_2.clh:14-41, observer_basic.clh:6-14,
+//  observer_basic_allVar.clh:6-27, observer_neighborhood_1.clh:6-37)
+std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, int nStore, unsigned int fVar)
 {
     std::vector<std::string> f;
     auto var = [&](int j) { return j < (int)pi.varNames.size() ? pi.varNames[j] : "x" + std::to_string(j); };
@@ -139,6 +141,37 @@ std::vector<std::string> observer_feature_names(int obs, const ProblemInfo &pi, 
             f.push_back("nhood event time " + std::to_string(j));
         for (const char *s : {"event count", "step count", "max dt", "min dt", "mean dt"})
             f.push_back(s);
+    } else if (obs == CLODE_OBS_BASIC) {
+        // basic reports the feature variable only
+        int k = (int)fVar;
+        f.push_back("max " + var(k));
+        f.push_back("min " + var(k));
+        f.push_back("mean " + var(k));
+        f.push_back("max d" + var(k) + "/dt");
+        f.push_back("min d" + var(k) + "/dt");
+        f.push_back("step count");
+    } else if (obs == CLODE_OBS_BASICALL || obs == CLODE_OBS_NHOOD1) {
+        if (obs == CLODE_OBS_NHOOD1)
+            for (const char *s : {"period", "peaks"})
+                for (const char *p : {"max ", "min ", "mean "})
+                    f.push_back(std::string(p) + s);
+        for (int j = 0; j < pi.nVar; ++j) {
+            f.push_back("max " + var(j));
+            f.push_back("min " + var(j));
+            f.push_back("mean " + var(j));
+            f.push_back("max d" + var(j) + "/dt");
+            f.push_back("min d" + var(j) + "/dt");
+        }
+        for (int j = 0; j < pi.nAux; ++j) {
+            f.push_back("max " + aux(j));
+            f.push_back("min " + aux(j));
+            f.push_back("mean " + aux(j));
+        }
+        if (obs == CLODE_OBS_NHOOD1)
+            for (const char *s : {"period count", "step count", "max dt", "min dt", "mean dt"})
+                f.push_back(s);
+        else
+            f.push_back("step count");
     }
     return f;
 }
@@ -217,8 +250,100 @@ std::string compose_source(const ProgramConfig &cfg)
     if (cfg.observer > 0)
         s << "#include \"clode_observers.h\"\n";
     s << "#include \"clode_kernels.h\"\n";
-    if (cfg.persistent)
-        s << "#include \"clode_persistent.h\"\n";
+    s << "#include \"clode_queue.h\"\n";
+    return s.str();
+}
+
+// One system of a batched program: its defines, right-hand side, steppers,
+// observer and drivers inside namespace clode_sys_<k>.
+static void compose_system(std::ostringstream &s, const ProgramConfig &cfg, size_t k)
+{
+    s << "namespace clode_sys_" << k << " {\n";
+    s << "#define CLODE_STEPPER " << cfg.stepper << "\n";
+    if (cfg.observer > 0) {
+        s << "#define CLODE_OBSERVER " << cfg.observer << "\n";
+        s << "#define CLODE_EVAR " << cfg.eVar << "\n";
+        s << "#define CLODE_FVAR " << cfg.fVar << "\n";
+    }
+    s << "#define N_VAR " << cfg.nVar << "\n";
+    s << "#define N_PAR " << cfg.nPar << "\n";
+    s << "#define N_AUX " << cfg.nAux << "\n";
+    s << "#define N_WIENER " << cfg.nWiener << "\n";
+    s << "#define N_STORE_EVENTS " << cfg.nStore << "\n";
+    s << "#pragma clang force_cuda_host_device begin\n";
+    std::string rname = cfg.rhs_name.empty() ? std::string("rhs.cl") : cfg.rhs_name;
+    size_t slash = rname.find_last_of('/');
+    if (slash != std::string::npos)
+        rname = rname.substr(slash + 1);
+    s << "#line 1 \"" << rname << "\"\n";
+    s << cfg.rhs << "\n";
+    s << "#pragma clang force_cuda_host_device end\n";
+    s << "#include \"clode_steppers.h\"\n";
+    if (cfg.observer > 0)
+        s << "#include \"clode_observers.h\"\n";
+    s << "#include \"clode_kernels.h\"\n";
+    s << "}  // namespace clode_sys_" << k << "\n";
+    s << "#include \"clode_system_end.h\"\n";
+}
+
+// Batched program: every system compiled once in its own namespace, then the
+// dispatching work-queue kernels.  Wave w starts on system w mod nsys and
+// visits the others in turn, so each wave works on one system at a time (no
+// divergence between systems inside a wave) and waves whose system's queue is
+// empty move on to help with the others.
+std::string compose_batch_source(const std::vector<ProgramConfig> &cfgs)
+{
+    if (cfgs.empty() || cfgs.size() > CLODE_BATCH_MAX)
+        throw InvalidArgument("a batch holds 1 to " + std::to_string(CLODE_BATCH_MAX) + " systems");
+    const int prec = cfgs[0].prec;
+    const bool obs = cfgs[0].observer > 0;
+    for (const auto &c : cfgs) {
+        if (c.prec != prec)
+            throw InvalidArgument("all systems of a batch must use the same precision");
+        if ((c.observer > 0) != obs)
+            throw InvalidArgument("all systems of a batch must be feature simulators, or none");
+    }
+    const size_t n = cfgs.size();
+    std::ostringstream s;
+    s << "// clode_amd hipRTC batch program\n";
+    s << "#define CLODE_PREC " << prec << "\n";
+    s << "#define CLODE_BATCH 1\n";
+    s << "#include \"clode_prelude.h\"\n";
+    s << "#include \"clode_kernel_args.h\"\n";
+    s << "#include \"clode_layout.h\"\n";
+    s << "#include \"clode_rng.h\"\n";
+    for (size_t k = 0; k < n; ++k)
+        compose_system(s, cfgs[k], k);
+    s << "#include \"clode_queue.h\"\n";
+    s << "#define CLODE_NSYS " << n << "u\n";
+    s << "struct ClodeBatchArgs {\n    ClodeArgs sys[CLODE_NSYS];\n};\n";
+    auto dispatcher = [&](const char *name, const char *lane) {
+        s << "extern \"C\" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_batch_" << name
+          << "(const ClodeBatchArgs b)\n{\n";
+        s << "    const unsigned int wave = (blockIdx.x * blockDim.x + threadIdx.x) / 64u;\n";
+        s << "    for (unsigned int k = 0; k < CLODE_NSYS; ++k) {\n";
+        s << "        switch ((wave + k) % CLODE_NSYS) {\n";
+        for (size_t j = 0; j < n; ++j)
+            s << "        case " << j << ": clode_sys_" << j << "::run_queue<clode_sys_" << j << "::" << lane
+              << ">(b.sys[" << j << "]); break;\n";
+        s << "        }\n    }\n}\n";
+    };
+    dispatcher("transient", "TransientLane");
+    if (obs) {
+        dispatcher("initialize_observer", "InitObsLane");
+        dispatcher("features", "FeaturesLane");
+    }
+    // x0 <- xf for every system, grid-stride
+    s << "extern \"C\" __global__ void __launch_bounds__(256) clode_batch_shift(const ClodeBatchArgs b)\n{\n";
+    s << "    const size_t stride = (size_t)gridDim.x * blockDim.x;\n";
+    s << "    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;\n";
+    const char *xt = prec == PREC_SINGLE ? "float" : "double";
+    for (size_t j = 0; j < n; ++j) {
+        s << "    for (size_t e = t; e < (size_t)" << cfgs[j].nVar << " * b.sys[" << j << "].nPts; e += stride)\n";
+        s << "        ((" << xt << " *)b.sys[" << j << "].x0)[e] = ((const " << xt << " *)b.sys[" << j
+          << "].xf)[e];\n";
+    }
+    s << "}\n";
     return s.str();
 }
 
@@ -305,9 +430,15 @@ static void rtc_check(hiprtcResult r, const char *what)
 
 std::shared_ptr<Program> get_program(const ProgramConfig &cfg, const std::string &cache_dir)
 {
+    return get_program_source(compose_source(cfg), compile_options(cfg), cache_dir);
+}
+
+std::shared_ptr<Program> get_program_source(const std::string &source, const std::vector<std::string> &options,
+                                            const std::string &cache_dir)
+{
     auto prog = std::make_shared<Program>();
-    prog->source = compose_source(cfg);
-    prog->options = compile_options(cfg);
+    prog->source = source;
+    prog->options = options;
     std::string keytext = prog->source;
     for (const auto &o : prog->options)
         keytext += "\n" + o;

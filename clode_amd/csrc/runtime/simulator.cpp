@@ -135,7 +135,6 @@ ProgramConfig Simulator::programConfig() const
     c.nAux = prob.nAux;
     c.nWiener = prob.nWiener;
     c.nStore = kind == 1 ? (int)op.maxEventTimestamps : 0;
-    c.persistent = 0;
     c.rhs = rhsSource;
     c.rhs_name = prob.src_file;
     return c;
@@ -248,6 +247,7 @@ void Simulator::setNpts(int newNpts)
     upload(d_dt, dtHost, wideSize());
     memberTspan = false;
     d_tspan_m.release();
+    d_qorder.release();
     d_steps.release();
     seedRNG();
     log_msg(LOG_DEBUG, "set nPts=" + std::to_string(nPts));
@@ -358,14 +358,29 @@ void Simulator::setMemberTspan(const double *t, size_t n)
     if (!t) {
         memberTspan = false;
         d_tspan_m.release();
+        d_qorder.release();
         return;
     }
     requireData("set_member_tspan");
     if (n != 2 * (size_t)nPts)
         throw InvalidArgument("member tspan must have 2*nPts elements");
+    for (size_t i = 0; i < (size_t)nPts; ++i)
+        if (!(t[nPts + i] >= t[i]))
+            throw InvalidArgument("member tspan: every end must be >= its start");
+    // work-queue order: longest span first, so that the longest members start
+    // first and the short ones fill the tail
+    std::vector<unsigned int> order((size_t)nPts);
+    for (size_t i = 0; i < order.size(); ++i)
+        order[i] = (unsigned int)i;
+    std::stable_sort(order.begin(), order.end(), [&](unsigned int a, unsigned int b) {
+        return t[nPts + a] - t[a] > t[nPts + b] - t[b];
+    });
     d_tspan_m.allocate(device, n * 8);
+    d_qorder.allocate(device, order.size() * 4);
     ensureStream();
     hip_check(hipMemcpyAsync(d_tspan_m.ptr, t, n * 8, hipMemcpyHostToDevice, stream), "hipMemcpy H2D");
+    hip_check(hipMemcpyAsync(d_qorder.ptr, order.data(), order.size() * 4, hipMemcpyHostToDevice, stream),
+              "hipMemcpy H2D");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     memberTspan = true;
 }
@@ -424,20 +439,59 @@ ClodeArgs Simulator::makeArgs()
         }
         a.steps = d_steps.as<unsigned long long>();
     }
-    a.member_offset = 0;
-    a.n_launch = (unsigned int)nPts;
+    a.q_order = memberTspan ? d_qorder.as<unsigned int>() : nullptr;
+    a.q_count = (unsigned int)nPts;
+    a.q_stride = 1;
+    a.q_slice = queueSlice;
     return a;
+}
+
+// Persistent work-queue grid: as many workgroups as can be resident at once
+// (occupancy of this kernel at this block size x compute units), but no more
+// than the members need.
+unsigned int queue_grid(hipFunction_t fn, int device, unsigned int block, size_t members)
+{
+    int perCU = 0, cus = 0;
+    hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, (int)block, 0),
+              "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    size_t resident = (size_t)std::max(perCU, 1) * (size_t)std::max(cus, 1);
+    size_t needed = (members + block - 1) / block;
+    return (unsigned int)std::max<size_t>(1, std::min(resident, needed));
 }
 
 void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads)
 {
     auto p = currentProgram();
-    hipFunction_t fn = p->function(device, kernel);
     ensureStream();
     unsigned int block = blockSize > 0 ? (unsigned int)blockSize : 64u;
-    unsigned int grid = (nThreads + block - 1) / block;
     ClodeArgs args = a;
     void *params[] = {&args};
+    if (launchMode == 1) {
+        // work queue: "clode_X" -> "clode_queue_X", counter cleared by a kernel
+        std::string qname = std::string("clode_queue_") + (kernel + 6);
+        hipFunction_t fn = p->function(device, qname.c_str());
+        if (!d_work.ptr)
+            d_work.allocate(device, 256);
+        args.work_counter = d_work.as<unsigned int>();
+        hipFunction_t reset = p->function(device, "clode_queue_reset");
+        unsigned int one = 1;
+        void *rparams[] = {&args.work_counter, &one};
+        hip_check(hipModuleLaunchKernel(reset, 1, 1, 1, 64, 1, 1, 0, stream, rparams, nullptr),
+                  "launch clode_queue_reset");
+        unsigned int grid = queue_grid(fn, device, block, nThreads);
+        hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+        hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr),
+                  ("launch " + qname).c_str());
+        hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+        hip_check(hipStreamSynchronize(stream), ("kernel " + qname).c_str());
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, ev0, ev1);
+        lastKernelMs = ms;
+        return;
+    }
+    hipFunction_t fn = p->function(device, kernel);
+    unsigned int grid = (nThreads + block - 1) / block;
     hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
     hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr),
               (std::string("launch ") + kernel).c_str());
@@ -537,7 +591,7 @@ void Simulator::updateObserverInfo()
     int obs = observerId();
     int S = (int)op.maxEventTimestamps;
     nFeatures = clode_obs_nfeatures(obs, prob.nVar, prob.nAux, S);
-    featureNames = observer_feature_names(obs, prob, S);
+    featureNames = observer_feature_names(obs, prob, S, op.fVarIx);
 }
 
 // CLODEfeatures::setObserverParams (CLODEfeatures.cpp:78-106)

@@ -92,6 +92,7 @@ public:
 
     int launchMode = 0;   // 0 grid, 1 persistent work queue
     int blockSize = 0;    // 0 = auto
+    unsigned int queueSlice = 8;  // work queue: loop iterations between queue checks
     bool dtRefresh = false;
     double lastKernelMs = 0.0;
     bool stepCounter = false;
@@ -101,10 +102,11 @@ public:
     std::vector<unsigned long long> getStepCounts();
 
 private:
+    friend class Batch;
     DeviceBuffer d_x0, d_pars, d_xf, d_rng, d_dt, d_tf;
     DeviceBuffer d_odr, d_odu, d_F;
     DeviceBuffer d_t, d_x, d_dx, d_aux, d_nStored;
-    DeviceBuffer d_work, d_tspan_m, d_steps;
+    DeviceBuffer d_work, d_tspan_m, d_steps, d_qorder;
     size_t odrFields = 0, oduFields = 0, Felements = 0;
     bool memberTspan = false;
     hipStream_t stream = nullptr;
@@ -128,5 +130,8 @@ private:
     void resizeTrajectoryVariables();
     void requireData(const char *what) const;
 };
+
+// resident-workgroup grid for a persistent work-queue launch of fn
+unsigned int queue_grid(hipFunction_t fn, int device, unsigned int block, size_t members);
 
 }  // namespace clode

@@ -356,10 +356,29 @@ class Simulator:
         self._runtime.print_devices()
 
     # --------------------------------------------------- extensions (native)
-    def set_launch(self, mode: str = "grid", block_size: int = 0) -> None:
+    def set_launch(self, mode: str = "grid", block_size: int = 0, slice: Optional[int] = None) -> None:
         """Work distribution: 'grid' (one member per lane) or 'persistent'
-        (lanes claim members from an atomic work counter)."""
+        (resident lanes claim the next member from an atomic work counter as
+        soon as theirs finishes; ``slice`` loop iterations between checks).
+        Results are identical in both modes."""
         self._integrator.set_launch({"grid": 0, "persistent": 1}[mode], block_size)
+        if slice is not None:
+            self._integrator.set_queue_slice(int(slice))
+
+    def set_member_tspan(self, t_start, t_end=None) -> None:
+        """Per-member time spans (no reference counterpart).  ``t_start`` and
+        ``t_end`` broadcast to the ensemble shape; ``set_member_tspan(None)``
+        returns to the global span.  The start times replace ``t_span[0]`` and
+        the end times ``t_span[1]`` member by member in every launch."""
+        if t_start is None:
+            self._integrator.set_member_tspan([])
+            return
+        n = self._ensemble_size
+        t0 = np.broadcast_to(np.asarray(t_start, dtype=np.float64), self._ensemble_shape).flatten(order="F")
+        t1 = np.broadcast_to(np.asarray(t_end, dtype=np.float64), self._ensemble_shape).flatten(order="F")
+        if t0.size != n or t1.size != n:
+            raise ValueError("member time spans must broadcast to the ensemble shape")
+        self._integrator.set_member_tspan(list(np.concatenate([t0, t1])))
 
     def last_kernel_ms(self) -> float:
         """Duration of the last kernel launch (HIP events), milliseconds."""

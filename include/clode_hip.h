@@ -183,12 +183,18 @@ int clode_set_log_level(int level);
 int clode_get_log_level(int *level);
 
 /* ---- extensions with no reference counterpart ----------------------- */
-/* Work distribution for features/transient launches: mode 0 = one member per
- * lane in a 1-D grid (the reference's NDRange), 1 = persistent grid whose
- * lanes claim members from an atomic work counter; block_size 0 = auto. */
+/* Work distribution for every launch: mode 0 = one member per lane in a 1-D
+ * grid (the reference's NDRange), 1 = persistent work queue: as many
+ * workgroups as fit on the device at once, whose lanes claim the next member
+ * with an atomic add as soon as their member finishes.  Results are identical
+ * in both modes.  block_size 0 = 64. */
 int clode_set_launch(clode_sim *sim, int mode, int block_size);
-/* Per-member end times (uneven spans, persistent mode): tspan_m is [2][nPts]
- * (t0 row then t1 row) or NULL to use the global span again. */
+/* Work-queue mode: loop iterations a lane runs between queue checks
+ * (default 8). */
+int clode_set_queue_slice(clode_sim *sim, int slice);
+/* Per-member time spans: tspan_m is [2][nPts] (start row, then end row) or
+ * NULL to use the global span again.  Every launch honours them; the work
+ * queue then hands out members longest span first. */
 int clode_set_member_tspan(clode_sim *sim, const double *tspan_m, size_t n);
 /* Refill the device step-size buffer with sp.dt on set_solver_params (the
  * reference refills it only when nPts changes). */
@@ -203,6 +209,42 @@ int clode_compile(clode_sim *sim);
 int clode_set_step_counter(clode_sim *sim, int enable);
 int clode_reset_step_counts(clode_sim *sim);
 int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n);
+
+/* ---- batches: several simulators in one work-queue launch ------------- */
+/* No reference counterpart (the reference runs one right-hand side per
+ * program and one time span per launch, clode/cpp/CLODE.cpp:148,
+ * transient.cl:10).  A batch holds 1 to 8 base or feature simulators of one
+ * precision on one device; their systems (right-hand sides, dimensions,
+ * steppers, observers) may all differ.  They are compiled into one program and
+ * integrated by dispatching work-queue kernels; each simulator keeps its own
+ * buffers and host semantics, and its results are bit-identical to running it
+ * alone.  The simulators must outlive the batch. */
+typedef struct clode_batch clode_batch;
+int clode_batch_create(clode_batch **out, clode_sim *const *sims, size_t n);
+int clode_batch_destroy(clode_batch *batch);
+/* compile-only check of the batched program for these simulators (no device
+ * needed) */
+int clode_batch_compile(clode_sim *const *sims, size_t n);
+/* workgroup size 64..256 (multiple of 64), loop iterations between queue
+ * checks >= 1; defaults 64 and 8 */
+int clode_batch_set_launch(clode_batch *batch, int block_size, int slice);
+int clode_batch_get_launch(clode_batch *batch, int *block_size, int *slice);
+/* transient / initialize_observer / features / shift_x0 of every simulator,
+ * each as one launch (features: reinitialize -1 continue or initialise where
+ * needed, 0 continue, 1 re-initialise) */
+int clode_batch_transient(clode_batch *batch);
+int clode_batch_initialize_observer(clode_batch *batch);
+int clode_batch_features(clode_batch *batch, int reinitialize);
+int clode_batch_shift_x0(clode_batch *batch);
+/* rounds x (transient, x0 <- xf, features, x0 <- xf); use_graph != 0 captures
+ * the whole sequence into one graph and launches it once */
+int clode_batch_sweep(clode_batch *batch, int rounds, int use_graph);
+/* Time the transient launch over every stride-th queued member for each
+ * block size in {64, 128, 256} and slice in {1, 4, 16, 64}, restoring the
+ * member state after each trial, and keep the fastest. */
+int clode_batch_autotune(clode_batch *batch, int stride, int *block_size, int *slice);
+/* device time of the last batch operation, milliseconds */
+int clode_batch_last_ms(clode_batch *batch, double *ms);
 
 #ifdef __cplusplus
 }

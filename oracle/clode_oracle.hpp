@@ -48,7 +48,7 @@
 namespace oracle {
 
 enum { ST_EULER = 1, ST_HEUN = 2, ST_RK4 = 3, ST_BS23 = 4, ST_DOPRI5 = 5, ST_SEULER = 6 };
-enum { OB_NONE = 0, OB_THRESH2 = 1, OB_LOCALMAX = 2, OB_NHOOD2 = 3 };
+enum { OB_NONE = 0, OB_THRESH2 = 1, OB_LOCALMAX = 2, OB_NHOOD2 = 3, OB_BASIC = 4, OB_BASICALL = 5, OB_NHOOD1 = 6 };
 
 template <class R>
 struct Consts;
@@ -374,9 +374,11 @@ inline R runningMeanTime(R mean, R newValue, R dt, R total_delta)
 }
 
 // ---------------------------------------------------------- observer data
-// Union of the fields of the three observers' ObserverData structs
+// Union of the fields of the observers' ObserverData structs
 // (observer_threshold_2.clh:63-116, observer_local_maximum.clh:54-87,
-// observer_neighborhood_2.clh:50-85); each observer touches its own subset.
+// observer_neighborhood_2.clh:50-85, observer_basic.clh:19-29,
+// observer_basic_allVar.clh:33-47, observer_neighborhood_1.clh:43-84); each
+// observer touches its own subset.
 template <class R>
 struct ObserverData {
     R tbuffer[3];
@@ -389,7 +391,7 @@ struct ObserverData {
     R IMI[3], amp[3];
     R fVarDownstateMean;
     R xGlobalMax, xGlobalMin, dxGlobalMax, dxGlobalMin, xUp, xDown, dxUp, dxDown;
-    R t_start, tLastEvent, tThisDown, tLastMax, tLastMin, xLastMin, xThreshold;
+    R t_start, t_last, tLastEvent, tThisDown, tLastMax, tLastMin, xLastMin, xThreshold;
     unsigned int thisNMaxima, stepcount, eventcount, inUpstate, foundX0, isInNhood;
 
     void resize(int N, int A, int S)
@@ -450,6 +452,7 @@ struct Observer {
         od.dxGlobalMin = big();
         od.xUp = od.xDown = od.dxUp = od.dxDown = 0;
         od.t_start = ti;
+        od.t_last = ti;
         od.tLastEvent = 0;
         od.tThisDown = 0;
         od.tLastMax = 0;
@@ -611,6 +614,64 @@ struct Observer {
                     od.list[3][od.eventcount - 1] = od.xLastMin;
                 }
             }
+        } else if (kind == OB_BASIC || kind == OB_BASICALL) {
+            // observer_basic.clh:63-73, observer_basic_allVar.clh:91-111: time
+            // means over the step since the previous update (t_last)
+            R dtb = ti - od.t_last;
+            od.t_last = ti;
+            R el = ti - od.t_start;
+            for (int j = 0; j < N; ++j) {
+                if (kind == OB_BASIC && j != (int)f)
+                    continue;
+                od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
+                od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+                od.xTrajectoryMean[j] = runningMeanTime(od.xTrajectoryMean[j], xi[j], dtb, el);
+                od.dxTrajectoryMax[j] = std::fmax(dxi[j], od.dxTrajectoryMax[j]);
+                od.dxTrajectoryMin[j] = std::fmin(dxi[j], od.dxTrajectoryMin[j]);
+            }
+            if (kind == OB_BASICALL)
+                for (int j = 0; j < A; ++j) {
+                    od.auxTrajectoryMax[j] = std::fmax(auxi[j], od.auxTrajectoryMax[j]);
+                    od.auxTrajectoryMin[j] = std::fmin(auxi[j], od.auxTrajectoryMin[j]);
+                    od.auxTrajectoryMean[j] = runningMeanTime(od.auxTrajectoryMean[j], auxi[j], dtb, el);
+                }
+        } else if (kind == OB_NHOOD1) {  // observer_neighborhood_1.clh:203-276
+            od.stepDt[0] = std::fmax(dt, od.stepDt[0]);
+            od.stepDt[1] = std::fmin(dt, od.stepDt[1]);
+            runningMean(&od.stepDt[2], dt, od.stepcount);
+            // step-count means, not time means
+            for (int j = 0; j < N; ++j) {
+                od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
+                od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+                runningMean(&od.xTrajectoryMean[j], xi[j], od.stepcount);
+                od.dxTrajectoryMax[j] = std::fmax(dxi[j], od.dxTrajectoryMax[j]);
+                od.dxTrajectoryMin[j] = std::fmin(dxi[j], od.dxTrajectoryMin[j]);
+            }
+            for (int j = 0; j < A; ++j) {
+                od.auxTrajectoryMax[j] = std::fmax(auxi[j], od.auxTrajectoryMax[j]);
+                od.auxTrajectoryMin[j] = std::fmin(auxi[j], od.auxTrajectoryMin[j]);
+                runningMean(&od.auxTrajectoryMean[j], auxi[j], od.stepcount);
+            }
+            if (od.stepcount > 1) {
+                if (!od.foundX0) {
+                    // first local minimum of the event variable is the centre
+                    if (od.dxbuffer[e * 3 + 1] <= 0.0 && od.dxbuffer[e * 3 + 2] > 0.0) {
+                        R mn = big();
+                        int ix = 0;
+                        for (int k = 0; k < 3; k++)
+                            if (od.xbuffer[e * 3 + k] < mn) {
+                                mn = od.xbuffer[e * 3 + k];
+                                ix = k;
+                            }
+                        od.tLastEvent = od.tbuffer[ix];
+                        od.foundX0 = 1;
+                        for (int j = 0; j < N; ++j)
+                            od.x0[j] = od.xbuffer[j * 3 + ix];
+                    }
+                } else if (od.dxbuffer[f * 3 + 1] >= 0.0 && od.dxbuffer[f * 3 + 2] < 0.0) {
+                    od.thisNMaxima++;
+                }
+            }
         } else if (kind == OB_NHOOD2) {  // :208-268
             od.stepDt[0] = std::fmax(dt, od.stepDt[0]);
             od.stepDt[1] = std::fmin(dt, od.stepDt[1]);
@@ -647,6 +708,22 @@ struct Observer {
             if (od.stepcount < 2)
                 return false;
             return (od.dxbuffer[f * 3 + 1] > 0.0 && od.dxbuffer[f * 3 + 2] < 0.0);
+        } else if (kind == OB_BASIC || kind == OB_BASICALL) {
+            return false;
+        } else if (kind == OB_NHOOD1) {  // observer_neighborhood_1.clh:148-168
+            if (od.stepcount < 2 || !od.foundX0)
+                return false;
+            if (od.xTrajectoryMax[f] - od.xTrajectoryMin[f] < op.minXamp)
+                return false;
+            unsigned int lastInNhood = od.isInNhood;
+            Buf<R> d;
+            for (int j = 0; j < N; ++j)
+                d[j] = std::fabs(xi[j] - od.x0[j]) / (od.xTrajectoryMax[j] - od.xTrajectoryMin[j]);
+            R result = 0;
+            for (int k = 0; k < N; k++)
+                result += d[k] * d[k];
+            od.isInNhood = std::sqrt(result) <= op.nHoodRadius;
+            return od.isInNhood && !lastInNhood;
         } else {  // nhood2 :157-175
             if (od.stepcount < 2 || !od.foundX0)
                 return false;
@@ -731,6 +808,23 @@ struct Observer {
                 od.list[1][od.eventcount - 1] = xThisMax;
             }
             return od.eventcount == op.maxEventCount;
+        } else if (kind == OB_BASIC || kind == OB_BASICALL) {
+            return false;
+        } else if (kind == OB_NHOOD1) {  // observer_neighborhood_1.clh:172-198
+            ++od.eventcount;
+            R tThisEvent = ti;
+            if (od.eventcount > 1) {
+                od.nMaxima[0] = std::fmax((R)od.thisNMaxima, od.nMaxima[0]);
+                od.nMaxima[1] = std::fmin((R)od.thisNMaxima, od.nMaxima[1]);
+                runningMean(&od.nMaxima[2], (R)od.thisNMaxima, od.eventcount - 1);
+                R thisPeriod = tThisEvent - od.tLastEvent;
+                od.period[0] = std::fmax(thisPeriod, od.period[0]);
+                od.period[1] = std::fmin(thisPeriod, od.period[1]);
+                runningMean(&od.period[2], thisPeriod, od.eventcount - 1);
+            }
+            od.tLastEvent = tThisEvent;
+            od.thisNMaxima = 0;
+            return od.eventcount >= op.maxEventCount;
         } else {  // nhood2 :178-205
             ++od.eventcount;
             R tThisEvent = ti;
@@ -808,6 +902,48 @@ struct Observer {
                     F.push_back(od.list[L][j]);
             F.push_back((R)od.eventcount);
             F.push_back((R)od.stepcount);
+        } else if (kind == OB_BASIC) {  // observer_basic.clh:77-85
+            const unsigned f = op.fVarIx;
+            F.push_back(od.xTrajectoryMax[f]);
+            F.push_back(od.xTrajectoryMin[f]);
+            F.push_back(od.xTrajectoryMean[f]);
+            F.push_back(od.dxTrajectoryMax[f]);
+            F.push_back(od.dxTrajectoryMin[f]);
+            F.push_back((R)od.stepcount);
+        } else if (kind == OB_BASICALL) {  // observer_basic_allVar.clh:114-130
+            for (int j = 0; j < N; ++j) {
+                F.push_back(od.xTrajectoryMax[j]);
+                F.push_back(od.xTrajectoryMin[j]);
+                F.push_back(od.xTrajectoryMean[j]);
+                F.push_back(od.dxTrajectoryMax[j]);
+                F.push_back(od.dxTrajectoryMin[j]);
+            }
+            for (int j = 0; j < A; ++j) {
+                F.push_back(od.auxTrajectoryMax[j]);
+                F.push_back(od.auxTrajectoryMin[j]);
+                F.push_back(od.auxTrajectoryMean[j]);
+            }
+            F.push_back((R)od.stepcount);
+        } else if (kind == OB_NHOOD1) {  // observer_neighborhood_1.clh:279-306
+            put3(od.period);
+            put3(od.nMaxima);
+            for (int j = 0; j < N; ++j) {
+                F.push_back(od.xTrajectoryMax[j]);
+                F.push_back(od.xTrajectoryMin[j]);
+                F.push_back(od.xTrajectoryMean[j]);
+                F.push_back(od.dxTrajectoryMax[j]);
+                F.push_back(od.dxTrajectoryMin[j]);
+            }
+            for (int j = 0; j < A; ++j) {
+                F.push_back(od.auxTrajectoryMax[j]);
+                F.push_back(od.auxTrajectoryMin[j]);
+                F.push_back(od.auxTrajectoryMean[j]);
+            }
+            F.push_back((R)(unsigned int)(od.eventcount - 1u));  // periods; wraps at eventcount 0
+            F.push_back((R)od.stepcount);
+            F.push_back(od.stepDt[0]);
+            F.push_back(od.stepDt[1]);
+            F.push_back(od.stepDt[2]);
         } else {  // nhood2 :271-302
             put3(od.period);
             put3(od.nMaxima);
@@ -848,7 +984,7 @@ struct Observer {
         } else if (kind == OB_LOCALMAX) {
             od.tLastMax = od.tLastMax - T;
             od.tLastMin = od.tLastMin - T;
-        } else {
+        } else if (kind == OB_NHOOD1 || kind == OB_NHOOD2) {
             od.tLastEvent -= T;
         }
         for (int j = 0; j < 3; ++j)
@@ -863,6 +999,12 @@ struct Observer {
             return 18 + 5 * N + 3 * A + 2 * S + 5;
         if (kind == OB_LOCALMAX)
             return 6 + 5 * N + 3 * A + 4 * S + 2;
+        if (kind == OB_BASIC)
+            return 6;
+        if (kind == OB_BASICALL)
+            return 5 * N + 3 * A + 1;
+        if (kind == OB_NHOOD1)
+            return 6 + 5 * N + 3 * A + 5;
         return 6 + 7 * N + 3 * A + S + 5;
     }
 };

@@ -145,6 +145,9 @@ struct Ensemble {
     std::vector<uint64_t> rng;
     R t0 = 0, t1 = 1000;
     double tspan_d[2] = {0, 1000};
+    std::vector<double> tspanM;  // optional per-member spans [2][nPts]: row 0 start, row 1 end
+    R mt0(int i) const { return tspanM.empty() ? t0 : (R)tspanM[i]; }
+    R mt1(int i) const { return tspanM.empty() ? t1 : (R)tspanM[(size_t)nPts + i]; }
     SolverParamsT<R> sp{};
     double sp_dt_d = 0.1;
     ObserverParamsT<R> op{};
@@ -199,6 +202,7 @@ struct Ensemble {
         F.clear();
         obsInit = false;
         storeAlloc = 0;
+        tspanM.clear();
     }
 
     struct Member {
@@ -215,7 +219,7 @@ struct Ensemble {
         m.dx.assign(N, 0);
         m.aux.assign(std::max(1, cfg.nAux), 0);
         m.w.assign(std::max(1, cfg.nWiener), 0);
-        m.ti = t0;
+        m.ti = mt0(i);
         m.dt = dt[i];
         for (int j = 0; j < cfg.nPar; ++j)
             m.p[j] = pars[(size_t)j * nPts + i];
@@ -246,6 +250,7 @@ struct Ensemble {
         Member m;
         load(m, i);
         pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
+        const R t1 = mt1(i);
         unsigned int step = 0;
         while (m.ti <= t1 && step < sp.max_steps) {
             ++step;
@@ -265,6 +270,7 @@ struct Ensemble {
         pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
         ObserverData<R> &d = od[i];
         ob.init(m.ti, m.x.data(), m.dx.data(), d);
+        const R t0 = mt0(i), t1 = mt1(i);
         unsigned int step = 0;
         if (ob.twoPass()) {
             while (m.ti < t1 && step < sp.max_steps) {
@@ -290,6 +296,7 @@ struct Ensemble {
         load(m, i);
         ObserverData<R> &d = od[i];
         pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
+        const R t0 = mt0(i), t1 = mt1(i);
         unsigned int step = 0;
         while (m.ti <= t1 && step < sp.max_steps) {
             ++step;
@@ -326,6 +333,7 @@ struct Ensemble {
             for (int j = 0; j < A; ++j)
                 taux[(s * A + j) * n + i] = m.aux[j];
         };
+        const R t1 = mt1(i);
         unsigned int storeix = 0;
         put(storeix);
         unsigned int step = 0;
@@ -597,6 +605,21 @@ int oracle_trajectory(void *h)
 {
     return with(h, [&](auto &e) {
         e.trajectory();
+        return 0;
+    });
+}
+
+// per-member time spans [2][nPts] (row 0 start, row 1 end); n == 0 clears
+int oracle_set_member_tspan(void *h, const double *t, size_t n)
+{
+    return with(h, [&](auto &e) {
+        if (n == 0) {
+            e.tspanM.clear();
+            return 0;
+        }
+        if (n != 2 * (size_t)e.nPts)
+            return 1;
+        e.tspanM.assign(t, t + n);
         return 0;
     });
 }

@@ -91,6 +91,13 @@ class Pair:
         if self.orc:
             self.orc.set_solver_params(*sp)
 
+    def set_member_tspan(self, tm):
+        """Per-member spans [2][n] (start row, end row) on both sides."""
+        tm = np.asarray(tm, dtype=np.float64)
+        self.gpu.set_member_tspan(list(tm.ravel()))
+        if self.orc:
+            self.orc.set_member_tspan(tm)
+
     def seed(self, s):
         self.gpu.seed_rng(s)
         if self.orc:

@@ -20,7 +20,7 @@ sys.path.insert(0, str(REPO / "oracle"))
 import build_oracle  # noqa: E402
 
 STEPPERS = {"euler": 1, "heun": 2, "rk4": 3, "bs23": 4, "dopri5": 5, "seuler": 6}
-OBSERVERS = {"": 0, "thresh2": 1, "localmax": 2, "nhood2": 3}
+OBSERVERS = {"": 0, "thresh2": 1, "localmax": 2, "nhood2": 3, "basic": 4, "basicall": 5, "nhood1": 6}
 
 _libs: dict[str, ctypes.CDLL] = {}
 
@@ -45,6 +45,7 @@ def _lib(model: str) -> ctypes.CDLL:
         lib.oracle_seed_rng.argtypes = [vp, ip]
         lib.oracle_set_rng.argtypes = [vp, u64p, sz]
         lib.oracle_set_dt.argtypes = [vp, dp, sz]
+        lib.oracle_set_member_tspan.argtypes = [vp, dp, sz]
         for f in ("oracle_transient", "oracle_initialize_observer", "oracle_trajectory", "oracle_shift_x0"):
             getattr(lib, f).argtypes = [vp]
         lib.oracle_features.argtypes = [vp, ip]
@@ -105,6 +106,14 @@ class OracleEnsemble:
 
     def set_tspan(self, tspan):
         self.lib.oracle_set_tspan(self.h, float(tspan[0]), float(tspan[1]))
+
+    def set_member_tspan(self, tspan_m):
+        """Per-member spans [2][nPts] (row 0 start, row 1 end); None clears."""
+        if tspan_m is None:
+            assert self.lib.oracle_set_member_tspan(self.h, None, 0) == 0
+            return
+        t = np.ascontiguousarray(tspan_m, dtype=np.float64).ravel()
+        assert self.lib.oracle_set_member_tspan(self.h, _dptr(t), t.size) == 0
 
     def set_solver_params(self, dt=0.1, dtmax=0.5, abstol=1e-6, reltol=1e-3, max_steps=1000000,
                           max_store=1000000, nout=1):

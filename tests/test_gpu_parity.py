@@ -51,6 +51,11 @@ for prec in ("double", "single"):
     for observer in ("thresh2", "localmax", "nhood2"):
         CASES.append(("lactotroph_noise", "seuler", observer, prec, 65, (0.0, 400.0),
                       (0.05, 1.0, 1e-6, 1e-3, 1000000, 1000000, 1)))
+    CASES.append(("van_der_pol", "rk4", "basicall", prec, 97, (0.0, 60.0), FIXED))
+    CASES.append(("lorenz", "bs23", "basic", prec, 130, (0.0, 40.0), ADAPT))
+    CASES.append(("lactotroph", "dopri5", "nhood1", prec, 70, (0.0, 1500.0), ADAPT))
+    CASES.append(("lactotroph_noise", "seuler", "basicall", prec, 65, (0.0, 400.0),
+                  (0.05, 1.0, 1e-6, 1e-3, 1000000, 1000000, 1)))
 
 
 def data_for(model, n):

@@ -62,7 +62,8 @@ def test_pybind_defaults_match_reference():
 
 @pytest.mark.parametrize("observer,nvar,naux,nstore", [
     ("thresh2", 4, 1, 0), ("thresh2", 2, 0, 3), ("localmax", 3, 0, 2), ("localmax", 3, 1, 0),
-    ("nhood2", 2, 0, 1), ("nhood2", 4, 2, 0)])
+    ("nhood2", 2, 0, 1), ("nhood2", 4, 2, 0), ("basic", 3, 1, 0), ("basicall", 4, 2, 0), ("basicall", 2, 0, 1),
+    ("nhood1", 3, 0, 0), ("nhood1", 4, 1, 2)])
 def test_feature_names_and_counts(observer, nvar, naux, nstore):
     from clode_amd.cpp import clode_cpp_wrapper as w
 
@@ -77,16 +78,22 @@ def test_feature_names_and_counts(observer, nvar, naux, nstore):
     names = s.get_feature_names()
     expected = {"thresh2": 18 + 5 * nvar + 3 * naux + 2 * nstore + 5,
                 "localmax": 6 + 5 * nvar + 3 * naux + 4 * nstore + 2,
-                "nhood2": 6 + 7 * nvar + 3 * naux + nstore + 5}[observer]
+                "nhood2": 6 + 7 * nvar + 3 * naux + nstore + 5, "basic": 6,
+                "basicall": 5 * nvar + 3 * naux + 1, "nhood1": 6 + 5 * nvar + 3 * naux + 5}[observer]
     assert s.get_n_features() == expected == len(names) == len(set(names))
-    assert names[-1 if observer == "localmax" else -4] == "step count"
+    assert names[-1 if observer in ("localmax", "basic", "basicall") else -4] == "step count"
+    if observer == "basic":  # the feature variable only (f_var_ix 0)
+        assert names[:5] == ["max v0", "min v0", "mean v0", "max dv0/dt", "min dv0/dt"]
+        return
     assert f"max d{vars_[-1]}/dt" in names
-    if nstore:
+    if observer == "nhood1":
+        assert names[-5] == "period count"
+    if nstore and observer in ("thresh2", "localmax", "nhood2"):
         key = {"thresh2": "up event time 0", "localmax": "localmax event evar 0", "nhood2": "nhood event time 0"}
         assert key[observer] in names
 
 
-@pytest.mark.parametrize("obs", [1, 2, 3])
+@pytest.mark.parametrize("obs", [1, 2, 3, 4, 5, 6])
 def test_observer_state_layout_matches_host(tmp_path, obs):
     """Compile the device observer header on the host and count the
     persistent fields it spills, against clode_layout.h (host allocation)."""
@@ -103,6 +110,8 @@ def test_observer_state_layout_matches_host(tmp_path, obs):
     ("features", "rk4", "thresh2", "double"), ("features", "dopri5", "thresh2", "fp32c"),
     ("features", "bs23", "localmax", "single"), ("features", "seuler", "nhood2", "double"),
     ("features", "heun", "nhood2", "fp32c"), ("features", "euler", "localmax", "fp32c"),
+    ("features", "rk4", "basic", "double"), ("features", "dopri5", "basicall", "fp32c"),
+    ("features", "bs23", "nhood1", "single"), ("features", "rk4", "nhood1", "fp32c"),
     ("trajectory", "dopri5", None, "single"), ("trajectory", "rk4", None, "fp32c"),
     ("base", "bs23", None, "double")])
 def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
@@ -160,3 +169,29 @@ def test_runtime_validation():
         initialize_runtime(None, None, None, None, [0])
     assert initialize_runtime(None, None, 0, 1, None).device == 1
     assert initialize_runtime(None, None, None, None, None).device == 0
+
+
+def test_batch_program_compiles_for_gfx950():
+    """Four different systems, steppers and observers in one batched program
+    (one namespace per system and dispatching work-queue kernels)."""
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    dev = w.OpenCLResource()
+    op = w.ObserverParams(max_event_timestamps=1)
+    sims = [
+        w.FeatureSimulatorBase(w.ProblemInfo(os.path.join(MODELS, "van_der_pol.cl"), ["x", "y"], ["mu"], [], 0),
+                               "rk4", "thresh2", op, "fp32c", dev, ""),
+        w.FeatureSimulatorBase(w.ProblemInfo(os.path.join(MODELS, "lorenz.cl"), ["x", "y", "z"], ["s", "r", "b"],
+                                             [], 0), "dopri5", "localmax", op, "fp32c", dev, ""),
+        w.FeatureSimulatorBase(w.ProblemInfo(os.path.join(MODELS, "chay_keizer.cl"), ["v", "n", "c"],
+                                             ["gca", "gkca", "kpmca"], [], 0), "bs23", "nhood2", op, "fp32c", dev, ""),
+        w.FeatureSimulatorBase(w.ProblemInfo(os.path.join(MODELS, "ornstein_uhlenbeck.cl"), ["x"],
+                                             ["theta", "mu", "sigma"], [], 1), "seuler", "basicall", op, "fp32c",
+                               dev, ""),
+    ]
+    w.compile_batch(sims)
+    mixed = [sims[0], w.FeatureSimulatorBase(
+        w.ProblemInfo(os.path.join(MODELS, "decay.cl"), ["x"], ["lambda"], [], 0), "rk4", "basic", op, "double",
+        dev, "")]
+    with pytest.raises(ValueError):
+        w.compile_batch(mixed)  # precisions differ

@@ -145,3 +145,54 @@ def test_localmax_no_events_for_monotone_decay():
     F = o.get("F")
     assert F[-2] == 0 and np.all(F[:6] == 0)
     assert F[-1] == 501 or F[-1] == 502  # step count (t <= 5 loop)
+
+
+def test_basicall_harmonic_closed_forms():
+    """x = A cos(w t), y = -A sin(w t) on [0, T]: extents +-A, slopes +-wA, and
+    time means A sin(wT)/(wT) and -A (1 - cos(wT))/(wT)."""
+    omega, A, T = 2.0, 1.5, 40.0
+    F = harmonic_features("basicall", omega, A)
+    assert len(F) == 5 * 2 + 1
+    assert F[0] == pytest.approx(A, abs=1e-6) and F[1] == pytest.approx(-A, abs=1e-5)
+    assert F[2] == pytest.approx(A * math.sin(omega * T) / (omega * T), abs=1e-4)
+    assert F[3] == pytest.approx(omega * A, abs=1e-5) and F[4] == pytest.approx(-omega * A, abs=1e-5)
+    assert F[5] == pytest.approx(A, abs=1e-5) and F[6] == pytest.approx(-A, abs=1e-5)
+    assert F[7] == pytest.approx(-A * (1 - math.cos(omega * T)) / (omega * T), abs=1e-4)
+    assert F[-1] in (40000, 40001, 40002)  # step count of the t <= T loop with dt 1e-3
+
+
+def test_basic_reports_the_feature_variable():
+    """basic with f_var_ix = 1 gives exactly basicall's block for y."""
+    Fb = harmonic_features("basic", obs=dict(f_var_ix=1))
+    Fa = harmonic_features("basicall")
+    assert len(Fb) == 6
+    assert np.array_equal(Fb[:5], Fa[5:10]) and Fb[5] == Fa[-1]
+
+
+def test_nhood1_harmonic_returns():
+    """Centre = state at the first minimum of x (t = pi/2); an event on every
+    entry into the ball: one just after the centre is found, then one per
+    return, entering 0.1 rad of phase (radius 0.05 of the range-normalised
+    circle of radius 0.5) before t = pi/2 + k pi, k = 1..12."""
+    omega = 2.0
+    F = harmonic_features("nhood1", omega)
+    names_block = 6 + 5 * 2
+    period_count, step_count = F[names_block], F[names_block + 1]
+    assert period_count == 12
+    assert F[0] == pytest.approx(math.pi, abs=2e-3)          # max period
+    assert F[1] == pytest.approx(math.pi - 0.05, abs=5e-3)   # first period: from the centre to the first entry
+    assert F[5] == pytest.approx(1.0)                        # one maximum of x per return
+    assert F[6] == pytest.approx(1.5, abs=1e-6)              # max x
+    assert step_count in (40000, 40001, 40002)
+    # step-count mean of x (uniform steps): A sin(wT)/(wT)
+    assert F[8] == pytest.approx(1.5 * math.sin(80.0) / 80.0, abs=1e-4)
+
+
+def test_nhood1_without_events_reports_wrapped_period_count():
+    o = feature_sim("decay", 1, 1, 0, "rk4", "nhood1", "double", (0.01, 0.01, 1e-6, 1e-3, 10 ** 6, 10 ** 6, 1))
+    o.set_problem_data(np.ones((1, 1)), np.ones((1, 1)))
+    o.set_tspan((0.0, 5.0))
+    o.features()
+    F = o.get("F")
+    assert np.all(F[:6] == 0)
+    assert F[6 + 5] == 2.0 ** 32 - 1  # eventcount - 1 as unsigned with no event
