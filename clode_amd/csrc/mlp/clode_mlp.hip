@@ -1,0 +1,1054 @@
+// MLP surrogate of a feature map: trained and evaluated with hand-written
+// BF16 matrix-core (MFMA) kernels accumulating in FP32, for gfx950.
+//
+// No counterpart in the reference.  The network maps 2 normalised parameters
+// to feature predictions:
+//   encode   x (2) -> 16 fixed Fourier features, sin and cos    -> 32
+//   layer 1  32  -> 256, SiLU
+//   layer 2  256 -> 256, SiLU
+//   layer 3  256 -> 256, SiLU
+//   layer 4  256 -> 256, SiLU
+//   heads    256 -> 4 heads x (log-period, amplitude, 16 class logits) = 72,
+//            padded to 128 output columns (the padding stays exactly zero)
+// Weights are kept transposed, Wt[out][in], as FP32 master copies (Adam) and
+// BF16 copies (MFMA operands).  Activations are BF16, pre-activations FP32.
+//
+// The 4 heads share the trunk and are trained on different bootstrap subsets
+// of every minibatch (a sample is seen by a head unless a hash of
+// (sample, head) is 0 mod 4), so their disagreement measures the network's
+// uncertainty.  Screening flags a grid cell for re-solving by the integrator
+// when the heads disagree, when the predicted class differs from a
+// neighbour's, and within a dilation radius of such cells.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/clode_mlp.h"
+
+namespace {
+
+constexpr int NFF = 16;         // Fourier frequencies
+constexpr int D0 = 2 * NFF;     // encoded input width
+constexpr int H = 256;          // hidden width
+constexpr int NHID = 4;         // hidden layers
+constexpr int NHEAD = 4;        // heads
+constexpr int NCLS = 16;        // classes
+constexpr int HW = 2 + NCLS;    // outputs per head
+constexpr int NOUT = NHEAD * HW;  // 72 used output columns
+constexpr int NOUTP = 128;      // padded output columns
+constexpr int NLAYER = NHID + 1;
+
+typedef uint16_t bf16_t;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+// round to nearest even (NaN kept quiet), identical on host and device
+__host__ __device__ inline bf16_t f2bf(float f)
+{
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u && (u & 0x007fffffu))
+        return (bf16_t)((u >> 16) | 0x40);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__host__ __device__ inline float bf2f(bf16_t h)
+{
+    uint32_t u = (uint32_t)h << 16;
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+__device__ inline float dsilu(float z)
+{
+    float s = 1.0f / (1.0f + __expf(-z));
+    return s * (1.0f + z * (1.0f - s));
+}
+
+// ---------------------------------------------------------------- GEMM
+// C[M x N] = A[M x K] * B[K x N] with arbitrary strides on both operands,
+// BF16 operands, FP32 accumulation on v_mfma_f32_16x16x32_bf16, and a fused
+// epilogue.  Workgroup tile 64 x 64 (4 waves of 32 x 32, each 2 x 2 MFMA
+// blocks), K step 32.  Both tiles are staged in LDS with k contiguous, so each
+// lane's 8-element operand is one 16-byte LDS read.
+enum Epi { EPI_FWD_SILU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_DSILU = 2, EPI_WGRAD = 3 };
+
+struct GemmArgs {
+    int M, N, K;
+    const bf16_t *A;
+    long long lda_m, lda_k;
+    const bf16_t *B;
+    long long ldb_k, ldb_n;
+    const float *bias;  // [N]                      (forward)
+    const float *Zin;   // [M][ldz] pre-activation  (EPI_BWD_DSILU)
+    float *Zout;        // [M][ldz] pre-activation  (forward)
+    bf16_t *Yout;       // [M][ldy]                 (forward SiLU, dsilu)
+    float *Cout;        // [M][ldc]                 (EPI_WGRAD)
+    bf16_t *YTout;      // [N][ldt] transposed copy of Yout (optional)
+    int ldz, ldy, ldc, ldt;
+    int ksplit;         // K range of split z (blockIdx.z): [z * ksplit, (z + 1) * ksplit); 0 = all of K
+    long long cstride;  // Cout offset between splits (EPI_WGRAD)
+};
+
+constexpr int TM = 64, TN = 64, TK = 32, KPAD = 8;
+
+template <int EPI>
+__global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t As[TM][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[TN][TK + KPAD];
+    const int t = threadIdx.x;
+    const int lane = t & 63, wave = t >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+    const int kbeg = g.ksplit > 0 ? (int)blockIdx.z * g.ksplit : 0;
+    const int kend = g.ksplit > 0 ? min(g.K, kbeg + g.ksplit) : g.K;
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+            acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    // loader mapping: thread t stages row/col (t >> 2), k (t & 3) * 8 .. + 7
+    const int lr = t >> 2, lk = (t & 3) * 8;
+    const bool a_vec = g.lda_k == 1 && (g.lda_m % 8) == 0 && ((uintptr_t)g.A & 15) == 0;
+    const bool b_vec = g.ldb_k == 1 && (g.ldb_n % 8) == 0 && ((uintptr_t)g.B & 15) == 0;
+    for (int k0 = kbeg; k0 < kend; k0 += TK) {
+        {
+            const int m = m0 + lr, k = k0 + lk;
+            u16x8 v;
+            if (a_vec && m < g.M && k + 8 <= kend) {
+                v = *(const u16x8 *)(g.A + (long long)m * g.lda_m + k);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    v[s] = (m < g.M && k + s < kend) ? g.A[(long long)m * g.lda_m + (long long)(k + s) * g.lda_k] : 0;
+            }
+            *(u16x8 *)&As[lr][lk] = v;
+        }
+        {
+            const int n = n0 + lr, k = k0 + lk;
+            u16x8 v;
+            if (b_vec && n < g.N && k + 8 <= kend) {
+                v = *(const u16x8 *)(g.B + (long long)n * g.ldb_n + k);
+            } else {
+#pragma unroll
+                for (int s = 0; s < 8; ++s)
+                    v[s] = (n < g.N && k + s < kend) ? g.B[(long long)(k + s) * g.ldb_k + (long long)n * g.ldb_n] : 0;
+            }
+            *(u16x8 *)&Bs[lr][lk] = v;
+        }
+        __syncthreads();
+        const int r = lane & 15, kk = (lane >> 4) * 8;
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            bf16x8 a = *(const bf16x8 *)&As[wm * 32 + i * 16 + r][kk];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                bf16x8 b = *(const bf16x8 *)&Bs[wn * 32 + j * 16 + r][kk];
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+            }
+        }
+        __syncthreads();
+    }
+    // epilogue: lane holds rows (lane >> 4) * 4 + q, column lane & 15 of each block
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + q;
+                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                if (m >= g.M || n >= g.N)
+                    continue;
+                float v = acc[i][j][q];
+                if (EPI == EPI_FWD_SILU || EPI == EPI_FWD_LINEAR) {
+                    v += g.bias[n];
+                    g.Zout[(long long)m * g.ldz + n] = v;
+                    if (EPI == EPI_FWD_SILU) {
+                        const bf16_t y = f2bf(silu(v));
+                        g.Yout[(long long)m * g.ldy + n] = y;
+                        if (g.YTout)
+                            g.YTout[(long long)n * g.ldt + m] = y;
+                    }
+                } else if (EPI == EPI_BWD_DSILU) {
+                    const bf16_t y = f2bf(v * dsilu(g.Zin[(long long)m * g.ldz + n]));
+                    g.Yout[(long long)m * g.ldy + n] = y;
+                    if (g.YTout)
+                        g.YTout[(long long)n * g.ldt + m] = y;
+                } else {
+                    g.Cout[(long long)blockIdx.z * g.cstride + (long long)m * g.ldc + n] = v;
+                }
+            }
+}
+
+// ---------------------------------------------------------------- encode
+// A0t (optional): transposed copy [32][ldt]
+__global__ void __launch_bounds__(256) mlp_encode(const float *X, int n, const float *Bff, bf16_t *A0, bf16_t *A0t,
+                                                  int ldt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float x0 = X[2 * i], x1 = X[2 * i + 1];
+    for (int f = 0; f < NFF; ++f) {
+        const float p = 6.283185307179586f * (Bff[f] * x0 + Bff[NFF + f] * x1);
+        const bf16_t s = f2bf(sinf(p)), c = f2bf(cosf(p));
+        A0[(long long)i * D0 + 2 * f] = s;
+        A0[(long long)i * D0 + 2 * f + 1] = c;
+        if (A0t) {
+            A0t[(long long)(2 * f) * ldt + i] = s;
+            A0t[(long long)(2 * f + 1) * ldt + i] = c;
+        }
+    }
+}
+
+// grid inputs: cell c = ix * ny + iy at (lo + (hi - lo) * i / (n - 1)), normalised to [0, 1]
+__global__ void __launch_bounds__(256) mlp_grid_inputs(long long c0, int n, int nx, int ny, float *X)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const long long c = c0 + i;
+    const int ix = (int)(c / ny), iy = (int)(c % ny);
+    X[2 * i] = nx > 1 ? (float)ix / (float)(nx - 1) : 0.f;
+    X[2 * i + 1] = ny > 1 ? (float)iy / (float)(ny - 1) : 0.f;
+}
+
+// ---------------------------------------------------------------- loss
+__host__ __device__ inline bool head_keeps(unsigned long long sample, int head)
+{
+    unsigned long long z = sample * 4ull + (unsigned long long)head + 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    z ^= z >> 31;
+    return (z & 3ull) != 0ull;
+}
+
+// Per sample and head: squared error of the two normalised regression
+// targets plus the softmax cross-entropy of the class, over the samples the
+// head keeps; the loss is the mean over the batch of the sum over heads.
+// dOut (BF16, [n][128]) is the gradient of that loss w.r.t. the head outputs.
+__global__ void __launch_bounds__(256) mlp_loss(const float *Zh, int n, const float *T, const int *cls,
+                                                const unsigned long long *ids, float inv_n, bf16_t *dOut,
+                                                bf16_t *dOutT, int ldt, float *loss)
+{
+    __shared__ float red[256];
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    float l = 0.f;
+    if (i < n) {
+        const float *z = Zh + (long long)i * NOUTP;
+        bf16_t *d = dOut + (long long)i * NOUTP;
+        const unsigned long long id = ids ? ids[i] : (unsigned long long)i;
+        for (int h = 0; h < NHEAD; ++h) {
+            const float *zh = z + h * HW;
+            bf16_t *dh = d + h * HW;
+            if (!head_keeps(id, h)) {
+                for (int c = 0; c < HW; ++c)
+                    dh[c] = 0;
+                continue;
+            }
+            const float e0 = zh[0] - T[2 * i], e1 = zh[1] - T[2 * i + 1];
+            l += e0 * e0 + e1 * e1;
+            dh[0] = f2bf(2.f * e0 * inv_n);
+            dh[1] = f2bf(2.f * e1 * inv_n);
+            float mx = zh[2];
+            for (int c = 1; c < NCLS; ++c)
+                mx = fmaxf(mx, zh[2 + c]);
+            float se = 0.f;
+            for (int c = 0; c < NCLS; ++c)
+                se += __expf(zh[2 + c] - mx);
+            const int y = cls[i];
+            l += (mx + __logf(se)) - zh[2 + y];
+            for (int c = 0; c < NCLS; ++c) {
+                const float p = __expf(zh[2 + c] - mx) / se;
+                dh[2 + c] = f2bf((p - (c == y ? 1.f : 0.f)) * inv_n);
+            }
+        }
+        for (int c = NOUT; c < NOUTP; ++c)
+            d[c] = 0;
+        for (int c = 0; c < NOUTP; ++c)
+            dOutT[(long long)c * ldt + i] = d[c];
+    }
+    red[threadIdx.x] = l;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        atomicAdd(loss, red[0] * inv_n);
+}
+
+// bias gradient: db[c] = sum_m dZ[m][c], read from the transposed copy
+// dZt[c][m] (BF16, row stride ld) so that each block reads one contiguous row
+__global__ void __launch_bounds__(256) mlp_colsum(const bf16_t *dZt, int n, int ncol, int ld, float *db)
+{
+    __shared__ float red[256];
+    const int c = blockIdx.x;
+    float s = 0.f;
+    for (int m = threadIdx.x; m < n; m += blockDim.x)
+        s += bf2f(dZt[(long long)c * ld + m]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k)
+            red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && c < ncol)
+        db[c] = red[0];
+}
+
+// sum of the split-K partial products: out[e] = sum_z part[z * count + e]
+__global__ void __launch_bounds__(256) mlp_split_sum(const float *part, int splits, long long count, float *out)
+{
+    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= count)
+        return;
+    float s = 0.f;
+    for (int z = 0; z < splits; ++z)
+        s += part[(long long)z * count + e];
+    out[e] = s;
+}
+
+// layer table for the Adam kernel: weights of layer l occupy
+// [woff, woff + in * out) as Wt[out][in]; the transposed BF16 copy keeps
+// W[in][out] at the same offsets
+struct LayerTable {
+    long long woff[NLAYER];
+    int in[NLAYER], out[NLAYER];
+};
+
+// Adam (Kingma & Ba 2015) on the FP32 master parameters, refreshing both BF16
+// copies of the weights; frozen entries (mask 0: head padding) stay put
+__global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float *m1, float *m2, const uint8_t *mask,
+                                                bf16_t *pbf, bf16_t *pbfT, const LayerTable lt, long long n, float lr,
+                                                float b1, float b2, float eps, float c1, float c2)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !mask[i])
+        return;
+    const float gi = g[i];
+    const float a = b1 * m1[i] + (1.f - b1) * gi;
+    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    m1[i] = a;
+    m2[i] = b;
+    const float v = p[i] - lr * (a * c1) / (sqrtf(b * c2) + eps);
+    p[i] = v;
+    const bf16_t b16 = f2bf(v);
+    pbf[i] = b16;
+    for (int l = 0; l < NLAYER; ++l) {
+        const long long r = i - lt.woff[l];
+        if (r >= 0 && r < (long long)lt.in[l] * lt.out[l]) {
+            const int o = (int)(r / lt.in[l]), k = (int)(r % lt.in[l]);
+            pbfT[lt.woff[l] + (long long)k * lt.out[l] + o] = b16;
+        }
+    }
+}
+
+// minibatch gather: row r of the batch is sample perm[r] of the training set
+__global__ void __launch_bounds__(256) mlp_gather(const unsigned int *perm, int nb, const float *Xa, const float *Ta,
+                                                  const int *Ca, float *X, float *T, int *C, unsigned long long *ids)
+{
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= nb)
+        return;
+    const unsigned int j = perm[r];
+    X[2 * r] = Xa[2 * j];
+    X[2 * r + 1] = Xa[2 * j + 1];
+    T[2 * r] = Ta[2 * j];
+    T[2 * r + 1] = Ta[2 * j + 1];
+    C[r] = Ca[j];
+    ids[r] = j;
+}
+
+// ---------------------------------------------------------------- screening
+// Per cell: each head's (log-period, amplitude) back to feature units and
+// class argmax; the ensemble prediction is the head mean (softmax mean for the
+// class).  flags bit 0: the heads disagree (relative spread above tol, or a
+// head's class differs from the ensemble class).
+__global__ void __launch_bounds__(256) mlp_predict_cells(const float *Zh, int n, const float *tmean,
+                                                         const float *tstd, float tol, float *period, float *amp,
+                                                         int *cls, uint8_t *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    const float *z = Zh + (long long)i * NOUTP;
+    float P[NHEAD], A[NHEAD], prob[NCLS];
+    int hc[NHEAD];
+    for (int c = 0; c < NCLS; ++c)
+        prob[c] = 0.f;
+    float pm = 0.f, am = 0.f;
+    for (int h = 0; h < NHEAD; ++h) {
+        const float *zh = z + h * HW;
+        P[h] = expf(zh[0] * tstd[0] + tmean[0]);
+        A[h] = zh[1] * tstd[1] + tmean[1];
+        pm += P[h];
+        am += A[h];
+        float mx = zh[2];
+        int am_i = 0;
+        for (int c = 1; c < NCLS; ++c)
+            if (zh[2 + c] > mx) {
+                mx = zh[2 + c];
+                am_i = c;
+            }
+        hc[h] = am_i;
+        float se = 0.f;
+        for (int c = 0; c < NCLS; ++c)
+            se += __expf(zh[2 + c] - mx);
+        for (int c = 0; c < NCLS; ++c)
+            prob[c] += __expf(zh[2 + c] - mx) / se;
+    }
+    pm *= 1.f / NHEAD;
+    am *= 1.f / NHEAD;
+    int best = 0;
+    for (int c = 1; c < NCLS; ++c)
+        if (prob[c] > prob[best])
+            best = c;
+    bool dis = false;
+    for (int h = 0; h < NHEAD; ++h) {
+        dis |= fabsf(P[h] - pm) > tol * fabsf(pm);
+        dis |= fabsf(A[h] - am) > tol * fabsf(am);
+        dis |= hc[h] != best;
+    }
+    period[i] = pm;
+    amp[i] = am;
+    cls[i] = best;
+    flags[i] = dis ? 1 : 0;
+}
+
+// flags bit 1: a 4-neighbour on the nx x ny grid has another class
+__global__ void __launch_bounds__(256) mlp_boundary(const int *cls, int nx, int ny, uint8_t *flags)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)nx * ny)
+        return;
+    const int ix = (int)(c / ny), iy = (int)(c % ny);
+    const int k = cls[c];
+    bool b = (ix > 0 && cls[c - ny] != k) || (ix + 1 < nx && cls[c + ny] != k) || (iy > 0 && cls[c - 1] != k) ||
+             (iy + 1 < ny && cls[c + 1] != k);
+    if (b)
+        flags[c] |= 2;
+}
+
+// one step of 3 x 3 dilation of "any flag" into bit 2 (out must not alias in)
+__global__ void __launch_bounds__(256) mlp_dilate(const uint8_t *in, int nx, int ny, uint8_t *out)
+{
+    const long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= (long long)nx * ny)
+        return;
+    const int ix = (int)(c / ny), iy = (int)(c % ny);
+    uint8_t f = in[c];
+    bool any = false;
+    for (int dx = -1; dx <= 1; ++dx)
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int jx = ix + dx, jy = iy + dy;
+            if (jx >= 0 && jx < nx && jy >= 0 && jy < ny && in[(long long)jx * ny + jy])
+                any = true;
+        }
+    out[c] = (uint8_t)(f | (any && !f ? 4 : 0));
+}
+
+// ---------------------------------------------------------------- host
+thread_local std::string g_err;
+
+struct MlpError : std::runtime_error {
+    using std::runtime_error::runtime_error;
+};
+
+void check(hipError_t e, const char *what)
+{
+    if (e != hipSuccess)
+        throw MlpError(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <class T>
+struct Dev {
+    T *p = nullptr;
+    size_t n = 0;
+    void alloc(size_t k)
+    {
+        if (k <= n && p)
+            return;
+        free();
+        check(hipMalloc((void **)&p, std::max<size_t>(k, 1) * sizeof(T)), "hipMalloc");
+        n = k;
+    }
+    void free()
+    {
+        if (p)
+            (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~Dev() { free(); }
+};
+
+struct Layer {
+    int in, out;        // out x in weights, stored transposed Wt[out][in]
+    size_t woff, boff;  // offsets in the flat parameter vector
+};
+
+}  // namespace
+
+struct clode_mlp {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Layer L[NLAYER];
+    size_t nparam = 0;
+    std::vector<float> Bff;       // [2][16] Fourier frequencies
+    float tmean[2] = {0, 0}, tstd[2] = {1, 1};
+    Dev<float> p, g, m1, m2, dBff, dT, dtm;
+    Dev<uint8_t> mask;
+    Dev<bf16_t> pbf, pbfT;  // BF16 weights Wt[out][in] and W[in][out]
+    // activations for a chunk of rows, and transposed copies [width][ldt]
+    // (ldt = rows rounded up to 8) for the weight-gradient products
+    Dev<bf16_t> A0, Y[NHID], dY[NHID], dOut;
+    Dev<bf16_t> A0t, Yt[NHID], dYt[NHID], dOutT;
+    Dev<float> Z[NHID], Zh, X, T, loss, wsp;
+    Dev<int> C;
+    Dev<unsigned long long> ids;
+    long long adamStep = 0;
+    double lastMs = 0;
+    size_t rows = 0;
+
+    void ensureRows(size_t n)
+    {
+        if (n <= rows)
+            return;
+        const size_t nt = (n + 7) & ~(size_t)7;
+        A0.alloc(n * D0);
+        A0t.alloc(nt * D0);
+        for (int l = 0; l < NHID; ++l) {
+            Y[l].alloc(n * H);
+            dY[l].alloc(n * H);
+            Yt[l].alloc(nt * H);
+            dYt[l].alloc(nt * H);
+            Z[l].alloc(n * H);
+        }
+        dOut.alloc(n * NOUTP);
+        dOutT.alloc(nt * NOUTP);
+        Zh.alloc(n * NOUTP);
+        X.alloc(n * 2);
+        T.alloc(n * 2);
+        C.alloc(n);
+        ids.alloc(n);
+        rows = n;
+    }
+};
+
+namespace {
+
+void gemm(clode_mlp *h, int epi, const GemmArgs &a)
+{
+    const int splits = a.ksplit > 0 ? (a.K + a.ksplit - 1) / a.ksplit : 1;
+    dim3 grid((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, splits);
+    switch (epi) {
+    case EPI_FWD_SILU: hipLaunchKernelGGL(mlp_gemm<EPI_FWD_SILU>, grid, dim3(256), 0, h->stream, a); break;
+    case EPI_FWD_LINEAR: hipLaunchKernelGGL(mlp_gemm<EPI_FWD_LINEAR>, grid, dim3(256), 0, h->stream, a); break;
+    case EPI_BWD_DSILU: hipLaunchKernelGGL(mlp_gemm<EPI_BWD_DSILU>, grid, dim3(256), 0, h->stream, a); break;
+    default: hipLaunchKernelGGL(mlp_gemm<EPI_WGRAD>, grid, dim3(256), 0, h->stream, a); break;
+    }
+    check(hipGetLastError(), "mlp_gemm launch");
+}
+
+GemmArgs ga() { GemmArgs a; std::memset(&a, 0, sizeof(a)); return a; }
+
+inline int ldT(int n) { return (n + 7) & ~7; }
+
+constexpr int KSPLIT_MAX = 32;    // partial products per weight gradient
+constexpr int KSPLIT_ROWS = 256;  // rows per partial product (at least)
+
+// forward over n rows already encoded in h->A0; with train, the activations'
+// transposed copies are written for backward
+void forward(clode_mlp *h, int n, bool train = false)
+{
+    const bf16_t *in = h->A0.p;
+    int kin = D0;
+    for (int l = 0; l < NHID; ++l) {
+        GemmArgs a = ga();
+        a.M = n; a.N = H; a.K = kin;
+        a.A = in; a.lda_m = kin; a.lda_k = 1;
+        a.B = h->pbf.p + h->L[l].woff; a.ldb_k = 1; a.ldb_n = kin;
+        a.bias = h->p.p + h->L[l].boff;
+        a.Zout = h->Z[l].p; a.ldz = H;
+        a.Yout = h->Y[l].p; a.ldy = H;
+        if (train) {
+            a.YTout = h->Yt[l].p;
+            a.ldt = ldT(n);
+        }
+        gemm(h, EPI_FWD_SILU, a);
+        in = h->Y[l].p;
+        kin = H;
+    }
+    GemmArgs a = ga();
+    a.M = n; a.N = NOUTP; a.K = H;
+    a.A = in; a.lda_m = H; a.lda_k = 1;
+    a.B = h->pbf.p + h->L[NHID].woff; a.ldb_k = 1; a.ldb_n = H;
+    a.bias = h->p.p + h->L[NHID].boff;
+    a.Zout = h->Zh.p; a.ldz = NOUTP;
+    gemm(h, EPI_FWD_LINEAR, a);
+}
+
+// backward from h->dOut / h->dOutT (BF16) into h->g; every product reads
+// operands that are contiguous along its reduction index
+void backward(clode_mlp *h, int n)
+{
+    const int lt = ldT(n);
+    const bf16_t *dz = h->dOut.p, *dzt = h->dOutT.p;
+    int nout = NOUTP;
+    for (int l = NHID; l >= 0; --l) {
+        const Layer &Ly = h->L[l];
+        const bf16_t *xt = l > 0 ? h->Yt[l - 1].p : h->A0t.p;
+        // dWt[o][i] = sum_m dz[m][o] x[m][i] = sum_m dzt[o][m] xt[i][m]
+        // split the batch (the reduction) so that the product has enough
+        // workgroups, then sum the partial products in a fixed order
+        GemmArgs w = ga();
+        w.M = Ly.out; w.N = Ly.in; w.K = n;
+        w.A = dzt; w.lda_m = lt; w.lda_k = 1;
+        w.B = xt; w.ldb_k = 1; w.ldb_n = lt;
+        const int splits = std::max(1, std::min(KSPLIT_MAX, n / KSPLIT_ROWS));
+        w.ksplit = ((n + splits - 1) / splits + TK - 1) / TK * TK;
+        const int used = (n + w.ksplit - 1) / w.ksplit;
+        const long long count = (long long)Ly.out * Ly.in;
+        w.Cout = h->wsp.p; w.ldc = Ly.in; w.cstride = count;
+        gemm(h, EPI_WGRAD, w);
+        hipLaunchKernelGGL(mlp_split_sum, dim3((unsigned int)((count + 255) / 256)), dim3(256), 0, h->stream, h->wsp.p,
+                           used, count, h->g.p + Ly.woff);
+        check(hipGetLastError(), "mlp_split_sum launch");
+        hipLaunchKernelGGL(mlp_colsum, dim3(Ly.out), dim3(256), 0, h->stream, dzt, n, Ly.out, lt, h->g.p + Ly.boff);
+        check(hipGetLastError(), "mlp_colsum launch");
+        if (l == 0)
+            break;
+        // dY[l-1][m][i] = (sum_o dz[m][o] W[i][o]) * silu'(Z[l-1][m][i])
+        GemmArgs d = ga();
+        d.M = n; d.N = Ly.in; d.K = Ly.out;
+        d.A = dz; d.lda_m = nout; d.lda_k = 1;
+        d.B = h->pbfT.p + Ly.woff; d.ldb_k = 1; d.ldb_n = Ly.out;
+        d.Zin = h->Z[l - 1].p; d.ldz = H;
+        d.Yout = h->dY[l - 1].p; d.ldy = H;
+        d.YTout = h->dYt[l - 1].p; d.ldt = lt;
+        gemm(h, EPI_BWD_DSILU, d);
+        dz = h->dY[l - 1].p;
+        dzt = h->dYt[l - 1].p;
+        nout = H;
+    }
+}
+
+void encode(clode_mlp *h, const float *Xdev, int n, bool train = false)
+{
+    hipLaunchKernelGGL(mlp_encode, dim3((n + 255) / 256), dim3(256), 0, h->stream, Xdev, n, h->dBff.p, h->A0.p,
+                       train ? h->A0t.p : nullptr, ldT(n));
+    check(hipGetLastError(), "mlp_encode launch");
+}
+
+void upload(clode_mlp *h, void *dst, const void *src, size_t bytes)
+{
+    check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, h->stream), "hipMemcpy H2D");
+}
+void download(clode_mlp *h, void *dst, const void *src, size_t bytes)
+{
+    check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, h->stream), "hipMemcpy D2H");
+    check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+void refresh_bf16(clode_mlp *h)
+{
+    std::vector<float> p(h->nparam);
+    download(h, p.data(), h->p.p, h->nparam * 4);
+    std::vector<bf16_t> b(h->nparam), bt(h->nparam, 0);
+    for (size_t i = 0; i < b.size(); ++i)
+        b[i] = f2bf(p[i]);
+    for (int l = 0; l < NLAYER; ++l) {
+        const Layer &Ly = h->L[l];
+        for (int o = 0; o < Ly.out; ++o)
+            for (int k = 0; k < Ly.in; ++k)
+                bt[Ly.woff + (size_t)k * Ly.out + o] = b[Ly.woff + (size_t)o * Ly.in + k];
+    }
+    upload(h, h->pbf.p, b.data(), b.size() * 2);
+    upload(h, h->pbfT.p, bt.data(), bt.size() * 2);
+    check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+}
+
+// loss and gradient over n rows (inputs X, targets T, classes C already on the device)
+float loss_grad(clode_mlp *h, int n)
+{
+    encode(h, h->X.p, n, true);
+    forward(h, n, true);
+    check(hipMemsetAsync(h->loss.p, 0, 4, h->stream), "hipMemset");
+    hipLaunchKernelGGL(mlp_loss, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, n, h->T.p, h->C.p, h->ids.p,
+                       1.0f / (float)n, h->dOut.p, h->dOutT.p, ldT(n), h->loss.p);
+    check(hipGetLastError(), "mlp_loss launch");
+    backward(h, n);
+    float l = 0;
+    download(h, &l, h->loss.p, 4);
+    return l;
+}
+
+#define MLP_TRY try {
+#define MLP_CATCH                        \
+    }                                    \
+    catch (const std::invalid_argument &e) \
+    {                                    \
+        g_err = e.what();                \
+        return CLODE_MLP_ERR_INVALID;    \
+    }                                    \
+    catch (const std::exception &e)      \
+    {                                    \
+        g_err = e.what();                \
+        return CLODE_MLP_ERR_RUNTIME;    \
+    }                                    \
+    return CLODE_MLP_OK;
+
+#define CHECK_H(h)                        \
+    if (!(h)) {                           \
+        g_err = "null surrogate handle";  \
+        return CLODE_MLP_ERR_INVALID;     \
+    }
+
+}  // namespace
+
+extern "C" {
+
+const char *clode_mlp_last_error(void) { return g_err.c_str(); }
+
+int clode_mlp_create(clode_mlp **out, int device, unsigned long long seed)
+{
+    if (!out) {
+        g_err = "null output handle";
+        return CLODE_MLP_ERR_INVALID;
+    }
+    *out = nullptr;
+    MLP_TRY
+    int nd = 0;
+    if (hipGetDeviceCount(&nd) != hipSuccess || nd <= 0)
+        throw MlpError("no HIP device is available");
+    if (device < 0 || device >= nd)
+        throw std::invalid_argument("device ordinal out of range");
+    check(hipSetDevice(device), "hipSetDevice");
+    auto *h = new clode_mlp;
+    try {
+        h->device = device;
+        check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
+        check(hipEventCreate(&h->ev0), "hipEventCreate");
+        check(hipEventCreate(&h->ev1), "hipEventCreate");
+        size_t off = 0;
+        int in = D0;
+        for (int l = 0; l < NLAYER; ++l) {
+            int o = l < NHID ? H : NOUTP;
+            h->L[l] = Layer{in, o, off, off + (size_t)in * o};
+            off += (size_t)in * o + o;
+            in = o;
+        }
+        h->nparam = off;
+        // He-style initialisation for SiLU layers, small heads; Fourier frequencies ~ N(0, 3^2)
+        std::mt19937_64 rng(seed);
+        std::normal_distribution<float> nd01(0.f, 1.f);
+        std::vector<float> p(h->nparam, 0.f);
+        std::vector<uint8_t> mask(h->nparam, 1);
+        for (int l = 0; l < NLAYER; ++l) {
+            const Layer &Ly = h->L[l];
+            float sd = l < NHID ? std::sqrt(2.f / Ly.in) : std::sqrt(1.f / Ly.in);
+            for (int o = 0; o < Ly.out; ++o)
+                for (int i = 0; i < Ly.in; ++i) {
+                    size_t k = Ly.woff + (size_t)o * Ly.in + i;
+                    if (l == NHID && o >= NOUT) {
+                        mask[k] = 0;  // padding columns stay zero
+                        continue;
+                    }
+                    p[k] = sd * nd01(rng);
+                }
+            if (l == NHID)
+                for (int o = NOUT; o < NOUTP; ++o)
+                    mask[Ly.boff + o] = 0;
+        }
+        h->Bff.resize(2 * NFF);
+        for (auto &b : h->Bff)
+            b = 3.f * nd01(rng);
+        h->p.alloc(h->nparam);
+        h->g.alloc(h->nparam);
+        h->m1.alloc(h->nparam);
+        h->m2.alloc(h->nparam);
+        h->mask.alloc(h->nparam);
+        h->pbf.alloc(h->nparam);
+        h->pbfT.alloc(h->nparam);
+        h->wsp.alloc((size_t)KSPLIT_MAX * H * H);  // the largest weight gradient, per split
+        h->dBff.alloc(2 * NFF);
+        h->dtm.alloc(4);
+        h->loss.alloc(1);
+        upload(h, h->p.p, p.data(), p.size() * 4);
+        upload(h, h->mask.p, mask.data(), mask.size());
+        upload(h, h->dBff.p, h->Bff.data(), h->Bff.size() * 4);
+        check(hipMemsetAsync(h->g.p, 0, h->nparam * 4, h->stream), "hipMemset");
+        check(hipMemsetAsync(h->m1.p, 0, h->nparam * 4, h->stream), "hipMemset");
+        check(hipMemsetAsync(h->m2.p, 0, h->nparam * 4, h->stream), "hipMemset");
+        refresh_bf16(h);
+    } catch (...) {
+        clode_mlp_destroy(h);
+        throw;
+    }
+    *out = h;
+    MLP_CATCH
+}
+
+int clode_mlp_destroy(clode_mlp *h)
+{
+    if (!h)
+        return CLODE_MLP_OK;
+    (void)hipSetDevice(h->device);
+    if (h->stream) {
+        (void)hipStreamSynchronize(h->stream);
+        (void)hipStreamDestroy(h->stream);
+    }
+    if (h->ev0)
+        (void)hipEventDestroy(h->ev0);
+    if (h->ev1)
+        (void)hipEventDestroy(h->ev1);
+    delete h;
+    return CLODE_MLP_OK;
+}
+
+int clode_mlp_n_params(clode_mlp *h, size_t *n)
+{
+    CHECK_H(h) MLP_TRY *n = h->nparam;
+    MLP_CATCH
+}
+
+int clode_mlp_get_params(clode_mlp *h, float *out, size_t n)
+{
+    CHECK_H(h) MLP_TRY
+    if (n != h->nparam)
+        throw std::invalid_argument("parameter vector has the wrong length");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    download(h, out, h->p.p, n * 4);
+    MLP_CATCH
+}
+
+int clode_mlp_set_params(clode_mlp *h, const float *in, size_t n)
+{
+    CHECK_H(h) MLP_TRY
+    if (n != h->nparam)
+        throw std::invalid_argument("parameter vector has the wrong length");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    upload(h, h->p.p, in, n * 4);
+    refresh_bf16(h);
+    MLP_CATCH
+}
+
+int clode_mlp_get_fourier(clode_mlp *h, float *out, size_t n)
+{
+    CHECK_H(h) MLP_TRY
+    if (n != 2 * NFF)
+        throw std::invalid_argument("Fourier matrix has 32 entries");
+    std::copy(h->Bff.begin(), h->Bff.end(), out);
+    MLP_CATCH
+}
+
+int clode_mlp_set_target_scaling(clode_mlp *h, const float *mean2, const float *std2)
+{
+    CHECK_H(h) MLP_TRY
+    for (int k = 0; k < 2; ++k) {
+        if (!(std2[k] > 0))
+            throw std::invalid_argument("target scales must be positive");
+        h->tmean[k] = mean2[k];
+        h->tstd[k] = std2[k];
+    }
+    MLP_CATCH
+}
+
+int clode_mlp_forward(clode_mlp *h, const float *X, size_t n, float *out, unsigned short *encoded)
+{
+    CHECK_H(h) MLP_TRY
+    if (n == 0 || n > (1u << 18))
+        throw std::invalid_argument("forward takes 1 to 262144 rows");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    h->ensureRows(n);
+    upload(h, h->X.p, X, n * 8);
+    encode(h, h->X.p, (int)n);
+    forward(h, (int)n);
+    std::vector<float> z(n * NOUTP);
+    download(h, z.data(), h->Zh.p, z.size() * 4);
+    for (size_t i = 0; i < n; ++i)
+        std::copy(z.begin() + i * NOUTP, z.begin() + i * NOUTP + NOUT, out + i * NOUT);
+    if (encoded)
+        download(h, encoded, h->A0.p, n * D0 * 2);
+    MLP_CATCH
+}
+
+int clode_mlp_loss_grad(clode_mlp *h, const float *X, const float *T, const int *cls, size_t n, float *loss,
+                        float *grad)
+{
+    CHECK_H(h) MLP_TRY
+    if (n == 0 || n > (1u << 16))
+        throw std::invalid_argument("loss_grad takes 1 to 65536 rows");
+    for (size_t i = 0; i < n; ++i)
+        if (cls[i] < 0 || cls[i] >= NCLS)
+            throw std::invalid_argument("class labels must be in 0..15");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    h->ensureRows(n);
+    upload(h, h->X.p, X, n * 8);
+    upload(h, h->T.p, T, n * 8);
+    upload(h, h->C.p, cls, n * 4);
+    std::vector<unsigned long long> ids(n);
+    for (size_t i = 0; i < n; ++i)
+        ids[i] = i;
+    upload(h, h->ids.p, ids.data(), n * 8);
+    *loss = loss_grad(h, (int)n);
+    if (grad)
+        download(h, grad, h->g.p, h->nparam * 4);
+    MLP_CATCH
+}
+
+int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls, size_t n, int epochs, int batch,
+                    float lr, unsigned long long seed, float *epoch_loss)
+{
+    CHECK_H(h) MLP_TRY
+    if (n < 1 || n > (1u << 24) || epochs < 1 || batch < 64 || batch > 65536)
+        throw std::invalid_argument("train: 1..16777216 samples, epochs >= 1, batch 64..65536");
+    for (size_t i = 0; i < n; ++i)
+        if (cls[i] < 0 || cls[i] >= NCLS)
+            throw std::invalid_argument("class labels must be in 0..15");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    // the training set stays on the device; each step gathers a shuffled
+    // minibatch into the activation buffers
+    Dev<float> Xall, Tall, eloss;
+    Dev<int> Call;
+    Dev<unsigned int> dperm;
+    Xall.alloc(n * 2);
+    Tall.alloc(n * 2);
+    Call.alloc(n);
+    dperm.alloc(n);
+    eloss.alloc(1);
+    upload(h, Xall.p, X, n * 8);
+    upload(h, Tall.p, T, n * 8);
+    upload(h, Call.p, cls, n * 4);
+    h->ensureRows((size_t)batch);
+    std::mt19937_64 rng(seed);
+    std::vector<unsigned int> perm(n);
+    for (size_t i = 0; i < n; ++i)
+        perm[i] = (unsigned int)i;
+    const size_t steps_per_epoch = (n + batch - 1) / batch;
+    const double total = (double)steps_per_epoch * epochs;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    LayerTable lt;
+    for (int l = 0; l < NLAYER; ++l) {
+        lt.woff[l] = (long long)h->L[l].woff;
+        lt.in[l] = h->L[l].in;
+        lt.out[l] = h->L[l].out;
+    }
+    long long step = 0;
+    check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    for (int e = 0; e < epochs; ++e) {
+        std::shuffle(perm.begin(), perm.end(), rng);
+        upload(h, dperm.p, perm.data(), n * 4);
+        check(hipMemsetAsync(eloss.p, 0, 4, h->stream), "hipMemset");
+        for (size_t s = 0; s < steps_per_epoch; ++s, ++step) {
+            const size_t b0 = s * (size_t)batch;
+            const int nb = (int)std::min<size_t>((size_t)batch, n - b0);
+            hipLaunchKernelGGL(mlp_gather, dim3((nb + 255) / 256), dim3(256), 0, h->stream, dperm.p + b0, nb,
+                               Xall.p, Tall.p, Call.p, h->X.p, h->T.p, h->C.p, h->ids.p);
+            check(hipGetLastError(), "mlp_gather launch");
+            encode(h, h->X.p, nb, true);
+            forward(h, nb, true);
+            hipLaunchKernelGGL(mlp_loss, dim3((nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->T.p,
+                               h->C.p, h->ids.p, 1.0f / (float)nb, h->dOut.p, h->dOutT.p, ldT(nb), eloss.p);
+            check(hipGetLastError(), "mlp_loss launch");
+            backward(h, nb);
+            // cosine decay to 5 % of the initial rate
+            const double frac = (double)step / total;
+            const float lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac))));
+            ++h->adamStep;
+            const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)h->adamStep)));
+            const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)h->adamStep)));
+            hipLaunchKernelGGL(mlp_adam, dim3((unsigned int)((h->nparam + 255) / 256)), dim3(256), 0, h->stream,
+                               h->p.p, h->g.p, h->m1.p, h->m2.p, h->mask.p, h->pbf.p, h->pbfT.p, lt,
+                               (long long)h->nparam, lr_t, b1, b2, eps, c1, c2);
+            check(hipGetLastError(), "mlp_adam launch");
+        }
+        float el = 0;
+        download(h, &el, eloss.p, 4);
+        if (epoch_loss)
+            epoch_loss[e] = el / (float)steps_per_epoch;
+    }
+    check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->lastMs = ms;
+    MLP_CATCH
+}
+
+int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, float *period, float *amp, int *cls,
+                          unsigned char *flags)
+{
+    CHECK_H(h) MLP_TRY
+    if (nx < 1 || ny < 1 || (long long)nx * ny > (1ll << 26))
+        throw std::invalid_argument("grid must have 1 to 2^26 cells");
+    if (dilate < 0 || dilate > 64)
+        throw std::invalid_argument("dilation radius must be 0..64");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    const long long ncell = (long long)nx * ny;
+    const int chunk = (int)std::min<long long>(ncell, 1 << 18);
+    h->ensureRows((size_t)chunk);
+    Dev<float> dP, dA;
+    Dev<int> dC;
+    Dev<uint8_t> dF, dF2;
+    dP.alloc(ncell);
+    dA.alloc(ncell);
+    dC.alloc(ncell);
+    dF.alloc(ncell);
+    dF2.alloc(ncell);
+    float tm[4] = {h->tmean[0], h->tmean[1], h->tstd[0], h->tstd[1]};
+    upload(h, h->dtm.p, tm, 16);
+    check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    for (long long c0 = 0; c0 < ncell; c0 += chunk) {
+        const int nb = (int)std::min<long long>(chunk, ncell - c0);
+        hipLaunchKernelGGL(mlp_grid_inputs, dim3((nb + 255) / 256), dim3(256), 0, h->stream, c0, nb, nx, ny, h->X.p);
+        check(hipGetLastError(), "mlp_grid_inputs launch");
+        encode(h, h->X.p, nb);
+        forward(h, nb);
+        hipLaunchKernelGGL(mlp_predict_cells, dim3((nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->dtm.p,
+                           h->dtm.p + 2, tol, dP.p + c0, dA.p + c0, dC.p + c0, dF.p + c0);
+        check(hipGetLastError(), "mlp_predict_cells launch");
+    }
+    const unsigned int gb = (unsigned int)((ncell + 255) / 256);
+    hipLaunchKernelGGL(mlp_boundary, dim3(gb), dim3(256), 0, h->stream, dC.p, nx, ny, dF.p);
+    check(hipGetLastError(), "mlp_boundary launch");
+    for (int r = 0; r < dilate; ++r) {
+        hipLaunchKernelGGL(mlp_dilate, dim3(gb), dim3(256), 0, h->stream, dF.p, nx, ny, dF2.p);
+        check(hipGetLastError(), "mlp_dilate launch");
+        std::swap(dF.p, dF2.p);
+    }
+    check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    download(h, period, dP.p, ncell * 4);
+    download(h, amp, dA.p, ncell * 4);
+    download(h, cls, dC.p, ncell * 4);
+    download(h, flags, dF.p, ncell);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->lastMs = ms;
+    MLP_CATCH
+}
+
+int clode_mlp_last_ms(clode_mlp *h, double *ms)
+{
+    CHECK_H(h) MLP_TRY *ms = h->lastMs;
+    MLP_CATCH
+}
+
+}  // extern "C"

@@ -12,6 +12,7 @@
 // ignored, device ids are HIP ordinals).  Precision: the reference's bool
 // cl_single_precision is accepted, and additionally the strings "single",
 // "double" and "fp32c" (float state with Kahan-compensated updates).
+#include <pybind11/numpy.h>
 #include <pybind11/pybind11.h>
 #include <pybind11/stl.h>
 
@@ -23,6 +24,10 @@
 #include "clode_hip.h"
 
 namespace py = pybind11;
+
+// 1-D float64 arrays: numpy arrays are used in place (lists and other
+// sequences are converted), results are returned as owning numpy arrays
+typedef py::array_t<double, py::array::c_style | py::array::forcecast> DArray;
 
 namespace {
 
@@ -272,13 +277,13 @@ public:
     void set_opencl_ids(unsigned int, unsigned int device) { check(clode_set_device(sim_, (int)device)); }
     void build_cl() { check(clode_build(sim_)); }
     void compile() { check(clode_compile(sim_)); }
-    void set_problem_data(const std::vector<double> &x0, const std::vector<double> &pars)
+    void set_problem_data(const DArray &x0, const DArray &pars)
     {
-        check(clode_set_problem_data(sim_, x0.data(), x0.size(), pars.data(), pars.size()));
+        check(clode_set_problem_data(sim_, x0.data(), (size_t)x0.size(), pars.data(), (size_t)pars.size()));
     }
     void set_tspan(const std::vector<double> &t) { check(clode_set_tspan(sim_, t.data(), t.size())); }
-    void set_x0(const std::vector<double> &x) { check(clode_set_x0(sim_, x.data(), x.size())); }
-    void set_pars(const std::vector<double> &p) { check(clode_set_pars(sim_, p.data(), p.size())); }
+    void set_x0(const DArray &x) { check(clode_set_x0(sim_, x.data(), (size_t)x.size())); }
+    void set_pars(const DArray &p) { check(clode_set_pars(sim_, p.data(), (size_t)p.size())); }
     void set_solver_params(const clode_solver_params &sp) { check(clode_set_solver_params(sim_, &sp)); }
     void seed_rng() { check(clode_seed_rng(sim_)); }
     void seed_rng_int(int seed) { check(clode_seed_rng_int(sim_, seed)); }
@@ -308,34 +313,34 @@ public:
         check(clode_get_n_pts(sim_, &n));
         return n;
     }
-    std::vector<double> get_pars()
+    DArray get_pars()
     {
-        std::vector<double> v((size_t)n_pts() * pi_.num_par);
-        check(clode_get_pars(sim_, v.data(), v.size()));
+        DArray v((size_t)n_pts() * pi_.num_par);
+        check(clode_get_pars(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_x0()
+    DArray get_x0()
     {
-        std::vector<double> v((size_t)n_pts() * pi_.num_var);
-        check(clode_get_x0(sim_, v.data(), v.size()));
+        DArray v((size_t)n_pts() * pi_.num_var);
+        check(clode_get_x0(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_xf()
+    DArray get_xf()
     {
-        std::vector<double> v((size_t)n_pts() * pi_.num_var);
-        check(clode_get_xf(sim_, v.data(), v.size()));
+        DArray v((size_t)n_pts() * pi_.num_var);
+        check(clode_get_xf(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_dt()
+    DArray get_dt()
     {
-        std::vector<double> v((size_t)n_pts());
-        check(clode_get_dt(sim_, v.data(), v.size()));
+        DArray v((size_t)n_pts());
+        check(clode_get_dt(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_tf()
+    DArray get_tf()
     {
-        std::vector<double> v((size_t)n_pts());
-        check(clode_get_tf(sim_, v.data(), v.size()));
+        DArray v((size_t)n_pts());
+        check(clode_get_tf(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
     std::vector<std::string> get_available_steppers()
@@ -351,9 +356,9 @@ public:
     void set_launch(int mode, int block) { check(clode_set_launch(sim_, mode, block)); }
     void set_queue_slice(int slice) { check(clode_set_queue_slice(sim_, slice)); }
     clode_sim *handle() const { return sim_; }
-    void set_member_tspan(const std::vector<double> &t)
+    void set_member_tspan(const DArray &t)
     {
-        check(clode_set_member_tspan(sim_, t.empty() ? nullptr : t.data(), t.size()));
+        check(clode_set_member_tspan(sim_, t.size() == 0 ? nullptr : t.data(), (size_t)t.size()));
     }
     void set_dt_refresh(bool on) { check(clode_set_dt_refresh(sim_, on ? 1 : 0)); }
     double last_kernel_ms()
@@ -429,10 +434,10 @@ public:
         check(clode_get_n_features(sim_, &n));
         return n;
     }
-    std::vector<double> get_f()
+    DArray get_f()
     {
-        std::vector<double> v((size_t)get_n_features() * n_pts());
-        check(clode_get_f(sim_, v.data(), v.size()));
+        DArray v((size_t)get_n_features() * n_pts());
+        check(clode_get_f(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
     std::vector<std::string> get_feature_names()
@@ -459,36 +464,36 @@ public:
         check(clode_trajectory(sim_));
     }
     void sizes(size_t &nt, size_t &nx, size_t &na) { check(clode_get_trajectory_sizes(sim_, &nt, &nx, &na)); }
-    std::vector<double> get_t()
+    DArray get_t()
     {
         size_t nt, nx, na;
         sizes(nt, nx, na);
-        std::vector<double> v(nt);
-        check(clode_get_t(sim_, v.data(), v.size()));
+        DArray v(nt);
+        check(clode_get_t(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_x()
+    DArray get_x()
     {
         size_t nt, nx, na;
         sizes(nt, nx, na);
-        std::vector<double> v(nx);
-        check(clode_get_x(sim_, v.data(), v.size()));
+        DArray v(nx);
+        check(clode_get_x(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_dx()
+    DArray get_dx()
     {
         size_t nt, nx, na;
         sizes(nt, nx, na);
-        std::vector<double> v(nx);
-        check(clode_get_dx(sim_, v.data(), v.size()));
+        DArray v(nx);
+        check(clode_get_dx(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
-    std::vector<double> get_aux()
+    DArray get_aux()
     {
         size_t nt, nx, na;
         sizes(nt, nx, na);
-        std::vector<double> v(na);
-        check(clode_get_aux(sim_, v.data(), v.size()));
+        DArray v(na);
+        check(clode_get_aux(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
     std::vector<int> get_n_stored()

@@ -1,0 +1,151 @@
+"""The BF16-MFMA surrogate network on the GPU against its CPU oracle, its
+training, its screening, and what it gains over the method without it.
+
+Tolerances: the oracle reproduces every BF16 rounding of the device and
+accumulates in float64 where the device accumulates in FP32, so outputs agree
+to 1e-4 in the median and 1e-2 at worst (where an FP32 sum lands on the other
+side of a BF16 rounding boundary, one activation moves by a BF16 ulp and the
+change propagates); gradients agree to 1 % in norm per layer with cosine
+> 0.9999.  A transposed or misindexed operand gives O(1) errors.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import mlp_oracle as mo
+
+pytestmark = pytest.mark.gpu
+
+MODELS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "clode_amd", "models")
+
+
+def net(seed=0):
+    from clode_amd.surrogate import MLPSurrogate
+
+    return MLPSurrogate(seed=seed)
+
+
+def test_forward_matches_oracle(gpu_available):
+    m = net(3)
+    rng = np.random.default_rng(0)
+    X = rng.random((1000, 2)).astype(np.float32)
+    out, enc = m.forward(X, with_encoding=True)
+    # the encoding: sin / cos of the Fourier projections, BF16, within one BF16 ulp
+    ref_enc = mo.bf16_round(mo.encode(X.astype(np.float64), m.fourier().astype(np.float64)))
+    got_enc = mo.bf16_bits_to_float(enc)
+    ulp = 2.0 ** (np.floor(np.log2(np.maximum(np.abs(ref_enc), 1e-30))) - 7)
+    assert np.all(np.abs(got_enc - ref_enc) <= ulp * 1.0001)
+    assert np.mean(got_enc == ref_enc) > 0.99
+    ref, _ = mo.forward(m.get_params().astype(np.float64), got_enc)
+    assert np.all(ref[:, mo.NOUT:] == 0)
+    err = np.abs(out - ref[:, :mo.NOUT])
+    assert np.max(err / (1 + np.abs(ref[:, :mo.NOUT]))) < 1e-2
+    assert np.median(err) < 1e-4
+
+
+def test_gradient_matches_oracle(gpu_available):
+    m = net(5)
+    rng = np.random.default_rng(1)
+    n = 3000  # not a multiple of the 64-row tile
+    X = rng.random((n, 2)).astype(np.float32)
+    T = rng.normal(0, 1, (n, 2)).astype(np.float32)
+    cls = rng.integers(0, 16, n)
+    loss, g = m.loss_grad(X, T, cls)
+    _, enc = m.forward(X, with_encoding=True)
+    rl, rg = mo.loss_grad(m.get_params().astype(np.float64), mo.bf16_bits_to_float(enc), T.astype(np.float64), cls)
+    assert abs(loss - rl) <= 1e-4 * abs(rl)
+    off = 0
+    for fin, fout in mo.LAYERS:
+        for part in (fin * fout, fout):
+            a, b = g[off:off + part].astype(np.float64), rg[off:off + part]
+            off += part
+            nb = np.linalg.norm(b)
+            assert np.linalg.norm(a - b) <= 1e-2 * nb + 1e-12, (fin, fout, part)
+            if nb > 0:
+                assert a @ b / (np.linalg.norm(a) * nb) > 0.9999
+    assert off == g.size
+
+
+def smooth_targets(U):
+    lp = np.sin(3 * U[:, 0]) + 0.5 * np.cos(2 * U[:, 1])
+    amp = U[:, 0] * U[:, 1] - 0.3
+    cls = np.clip((4 * U[:, 0] + 2 * U[:, 1]).astype(int), 0, 15)
+    return np.stack([lp, amp], axis=1), cls
+
+
+def test_training_fits_a_smooth_map(gpu_available):
+    m = net(7)
+    rng = np.random.default_rng(2)
+    U = rng.random((16384, 2))
+    T, cls = smooth_targets(U)
+    hist = m.train(U, T, cls, epochs=30, batch=1024, lr=2e-3, seed=1)
+    assert np.all(np.isfinite(hist)) and hist[-1] < hist[0] / 10
+    V = rng.random((4000, 2))
+    Tv, cv = smooth_targets(V)
+    out = m.forward(V)
+    for h in range(4):
+        z = out[:, h * mo.HW:(h + 1) * mo.HW]
+        assert np.median(np.abs(z[:, :2] - Tv)) < 0.03
+        assert np.mean(np.argmax(z[:, 2:], axis=1) == cv) > 0.95
+
+
+def test_screen_flags_class_boundary(gpu_available):
+    m = net(9)
+    rng = np.random.default_rng(3)
+    U = rng.random((16384, 2))
+    T = np.zeros((U.shape[0], 2))
+    cls = (U[:, 0] > 0.5).astype(int) * 3
+    m.set_target_scaling([0.0, 1.0], [1.0, 1.0])
+    m.train(U, T, cls, epochs=30, batch=1024, lr=2e-3, seed=4)
+    P, A, C, F = m.screen_grid(101, 37, tol=0.05, dilate=1)
+    assert P.shape == (101, 37)
+    assert np.all(C[:45] == 0) and np.all(C[56:] == 3)
+    # the boundary (x = 0.5, cell 50) is flagged as a class change in every
+    # column, and nowhere far from it
+    assert np.all((F[48:53] & 2).max(axis=0) > 0)
+    assert np.all((F[:40] & 2) == 0) and np.all((F[61:] & 2) == 0)
+    assert set(np.unique(F)) <= {0, 1, 2, 3, 4, 5, 6, 7}
+
+
+def lactotroph_solver(n_axis_check=None):
+    import clode_amd as clode
+    from clode_amd.surrogate import integrator_solver
+
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lactotroph.cl"),
+                                 variables={"v": -60.0, "n": 0.0, "f": 0.0, "c": 0.1},
+                                 parameters={"gcal": 1.5, "gsk": 3.0, "gbk": 1.0, "gk": 2.0}, aux=["ica"],
+                                 observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                 single_precision="fp32c", dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6,
+                                 max_steps=10 ** 8, observer_x_up_thresh=0.3, observer_x_down_thresh=0.2)
+    return integrator_solver(sim, ("gk", "gbk"), ((0.0, 10.0), (0.0, 2.0)), 1000.0, 6000.0, "v")
+
+
+def test_screened_map_and_ablation(gpu_available):
+    """On a 128 x 128 lactotroph map (fp32c): the trained surrogate re-solves
+    a minority of cells and the final map disagrees with the plain map on at
+    most 1 % of cells; an untrained network (the method without the learned
+    part) has to re-solve nearly everything to reach the same map."""
+    from clode_amd.surrogate import compare_maps, screened_map
+
+    solve = lactotroph_solver()
+    nx = ny = 128
+    gx, gy = np.meshgrid(np.arange(nx) / (nx - 1), np.arange(ny) / (ny - 1), indexing="ij")
+    plain = solve(np.stack([gx.ravel(), gy.ravel()], axis=1))
+    plain = {k: v.reshape(nx, ny) for k, v in plain.items()}
+    m, net_ = screened_map(solve, nx, ny, n_train=8192, epochs=60, batch=512, lr=2e-3, tol=0.05, dilate=1,
+                           fix_rounds=3, seed=1)
+    bad = compare_maps({"period": m.period, "amp": m.amp, "cls": m.cls}, plain, tol=0.05)
+    frac_solved = m.solved.mean()
+    assert bad.mean() <= 0.01, (bad.mean(), frac_solved)
+    assert frac_solved < 0.6, frac_solved
+    # without training: heads disagree almost everywhere
+    from clode_amd.surrogate import MLPSurrogate
+
+    raw = MLPSurrogate(seed=11)
+    raw.set_target_scaling(net_.target_mean, net_.target_std)
+    P, A, C, F = raw.screen_grid(nx, ny, tol=0.05, dilate=1)
+    untrained_flagged = (F != 0).mean()
+    unflagged = F == 0
+    untrained_bad = compare_maps({"period": P.astype(float), "amp": A.astype(float), "cls": C}, plain)[unflagged]
+    assert untrained_flagged > 0.9 or untrained_bad.mean() > 0.2
