@@ -165,30 +165,230 @@ def _cpu_model():
     return "unknown"
 
 
-def lorenz_ratio(n=1 << 20, t_end=500.0):
-    """Lorenz, 2^20 members, dopri5 + localmax: fp32c vs FP64 kernel time."""
+def vdp_c1(n=4096, cpu_members=256):
+    """Van der Pol, 4096 members, mu in [0.1, 5], rk4 dt 0.01, FP64: transient
+    over 1e3, x0 <- xf, thresh2 (warm-up + features) over 1e3.  GPU
+    member-steps/s, and the CPU oracle on every (n / cpu_members)-th member."""
+    import clode_amd as clode
+
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"), variables={"x": 2.0, "y": 0.0},
+                                 parameters={"mu": 1.0}, observer=clode.Observer.threshold_2,
+                                 stepper=clode.Stepper.rk4, single_precision=False, dt=0.01, dtmax=0.01,
+                                 max_steps=10 ** 8, t_span=(0.0, 1000.0), observer_x_up_thresh=0.3,
+                                 observer_x_down_thresh=0.2)
+    mu = 0.1 + 4.9 * np.arange(n) / (n - 1)
+    sim.set_ensemble(parameters={"mu": mu})
+    sim.enable_step_counter(True)
+    sim.transient(t_span=(0.0, 1000.0))  # warm (compile, first launch)
+    sim.set_ensemble(parameters={"mu": mu})
+    sim.reset_step_counts()
+    t0 = time.perf_counter()
+    sim.transient(t_span=(0.0, 1000.0))
+    out = sim.features(t_span=(0.0, 1000.0))
+    wall = time.perf_counter() - t0
+    steps = int(sim.get_step_counts().sum())
+    res = {"members": n, "member_steps": steps, "wall_s": wall, "member_steps_per_s": steps / wall,
+           "mean_period_mu1": float(np.interp(1.0, mu, np.ravel(out.get_var_max("period"))))}
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    from oracle import OracleEnsemble
+
+    idx = np.arange(0, n, n // cpu_members)
+    threads = min(os.cpu_count() or 1, 16)
+    o = OracleEnsemble("van_der_pol", 2, 1, 0, 0, stepper="rk4", observer="thresh2", precision="double",
+                       threads=threads)
+    o.set_solver_params(0.01, 0.01, 1e-6, 1e-3, 10 ** 8, 1000000, 1)
+    o.set_observer_params(x_up_threshold=0.3, x_down_threshold=0.2)
+    o.set_problem_data(np.vstack([2.0 * np.ones(idx.size), np.zeros(idx.size)]), mu[idx][None, :])
+    t0 = time.perf_counter()
+    o.set_tspan((0.0, 1000.0))
+    o.transient()
+    cs = o.last_steps
+    o.shift_x0()
+    o.features(1)
+    cs += o.last_steps + o.last_warm_steps
+    cw = time.perf_counter() - t0
+    res["cpu_oracle"] = {"members": int(idx.size), "threads": threads, "member_steps_per_s": cs / cw}
+    return res
+
+
+def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
+    """Lorenz, 2^20 members (sigma 10, beta 8/3, rho in [0, 250], x0 ~ N(1, 0.1)),
+    localmax on x over (0, 500), dopri5 and bs23 at rtol 1e-6 atol 1e-8, FP64
+    and fp32c; then the trajectory of the first 65,536 members, 1024 samples
+    each (nout 1), float state, with its HBM write rate."""
     import clode_amd as clode
 
     rng = np.random.default_rng(20240601)
     x0 = rng.normal(1.0, 0.1, size=(n, 3))
     rho = np.linspace(0.0, 250.0, n)
     res = {}
-    for prec in ("double", "fp32c"):
-        sim = clode.FeatureSimulator(
-            src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 1.0, "z": 1.0},
-            parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0}, observer=clode.Observer.local_max,
-            stepper=clode.Stepper.dormand_prince, single_precision=prec, dt=0.01, dtmax=1.0, abstol=1e-8,
-            reltol=1e-6, max_steps=100000000, t_span=(0.0, t_end))
-        sim.set_ensemble(variables=x0, parameters={"rho": rho})
-        sim.enable_step_counter(True)
-        t0 = time.perf_counter()
-        sim.features(fetch_results=False)
-        wall = time.perf_counter() - t0
-        steps = int(sim.get_step_counts().sum())
-        res[prec] = {"kernel_ms": sim.last_kernel_ms(), "wall_s": wall, "member_steps": steps,
-                     "member_steps_per_s": steps / wall}
-    res["fp32c_speedup_kernel"] = res["double"]["kernel_ms"] / res["fp32c"]["kernel_ms"]
+    common = dict(src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 1.0, "z": 1.0},
+                  parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0}, dt=0.01, dtmax=1.0, abstol=1e-8,
+                  reltol=1e-6, max_steps=100000000, t_span=(0.0, t_end))
+    for stepper in (clode.Stepper.dormand_prince, clode.Stepper.bs23):
+        r = {}
+        for prec in ("double", "fp32c"):
+            sim = clode.FeatureSimulator(observer=clode.Observer.local_max, stepper=stepper, single_precision=prec,
+                                         **common)
+            sim.set_ensemble(variables=x0, parameters={"rho": rho})
+            sim.enable_step_counter(True)
+            t0 = time.perf_counter()
+            out = sim.features(fetch_results=True)
+            wall = time.perf_counter() - t0
+            steps = int(sim.get_step_counts().sum())
+            r[prec] = {"kernel_ms": sim.last_kernel_ms(), "wall_s": wall, "member_steps": steps,
+                       "member_steps_per_s": steps / wall}
+            r[prec + "_F"] = out.to_ndarray()
+            r["names"] = out.get_feature_names()
+        # fp32c feature error on the members without chaotic transients (rho < 13)
+        calm = rho < 13.0
+        err = {}
+        for key in ("mean x", "max z", "min y"):
+            j = r["names"].index(key)
+            a, b = r["fp32c_F"][calm, j], r["double_F"][calm, j]
+            rel = np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+            err[key] = {"median_rel": float(np.median(rel)), "p99_rel": float(np.percentile(rel, 99))}
+        r["fp32c_feature_error_rho_lt_13"] = err
+        for k in ("fp32c_F", "double_F", "names"):
+            r.pop(k)
+        r["fp32c_speedup_kernel"] = r["double"]["kernel_ms"] / r["fp32c"]["kernel_ms"]
+        res[stepper.value] = r
+    sim = clode.TrajectorySimulator(stepper=clode.Stepper.dormand_prince, single_precision="fp32c",
+                                    max_store=store, nout=1, **common)
+    sim.set_ensemble(variables=x0[:n_traj], parameters={"rho": rho[:n_traj]})
+    sim.trajectory(fetch_results=False, update_x0=False)  # first launch (program load), untimed
+    sim.trajectory(fetch_results=False, update_x0=False)
+    ms = sim.last_kernel_ms()
+    ns = np.array(sim._integrator.get_n_stored(), dtype=np.int64)
+    samples = int(np.minimum(ns, store).sum())
+    per_sample = 8 + 2 * 3 * 4  # double time + x, dx (float)
+    gbs = samples * per_sample / (ms / 1e3) / 1e9
+    res["trajectory_65536x1024_fp32c"] = {"kernel_ms": ms, "samples": samples, "bytes_per_sample": per_sample,
+                                          "write_GBps": gbs, "frac_of_hbm_peak": gbs / 8000.0}
     return res
+
+
+def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
+    """Four systems in equal shares (2^20 members): Van der Pol (dopri5,
+    thresh2), Lorenz (dopri5, localmax), Chay-Keizer (dopri5, thresh2),
+    Ornstein-Uhlenbeck (theta 1, sigma 0.5, seuler dt 0.05, basicall), all
+    fp32c, per-member spans (0, 500 exp(z)), z ~ N(0, 1); rounds x (transient,
+    x0 <- xf, features, x0 <- xf) as one captured graph of batched work-queue
+    launches with autotuned launch shape, against the same sequence run system
+    by system with one-member-per-lane grid launches."""
+    import clode_amd as clode
+
+    rng = np.random.default_rng(seed)
+
+    def build():
+        sims = []
+        common = dict(single_precision="fp32c", max_steps=10 ** 9, abstol=1e-8, reltol=1e-6, dt=0.01, dtmax=1.0)
+        vdp = clode.FeatureSimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"),
+                                     variables={"x": 2.0, "y": 0.0}, parameters={"mu": 1.0},
+                                     observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                     observer_x_up_thresh=0.3, observer_x_down_thresh=0.2, **common)
+        vdp.set_ensemble(parameters={"mu": rng.uniform(0.1, 5.0, n_each)})
+        lor = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lorenz.cl"),
+                                     variables={"x": 1.0, "y": 1.0, "z": 1.0},
+                                     parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0},
+                                     observer=clode.Observer.local_max, stepper=clode.Stepper.dormand_prince, **common)
+        lor.set_ensemble(variables=rng.normal(1.0, 0.1, (n_each, 3)), parameters={"rho": rng.uniform(0, 50, n_each)})
+        ck = clode.FeatureSimulator(src_file=os.path.join(MODELS, "chay_keizer.cl"),
+                                    variables={"v": -50.0, "n": 0.01, "c": 0.12},
+                                    parameters={"gca": 1200.0, "gkca": 750.0, "kpmca": 0.1},
+                                    observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                    observer_x_up_thresh=0.3, observer_x_down_thresh=0.2, **common)
+        ck.set_ensemble(parameters={"gca": rng.uniform(550, 1050, n_each), "kpmca": rng.uniform(0.095, 0.155, n_each)})
+        ou = clode.FeatureSimulator(src_file=os.path.join(MODELS, "ornstein_uhlenbeck.cl"), variables={"x": 0.0},
+                                    parameters={"theta": 1.0, "mu": 0.0, "sigma": 0.5}, num_noise=1,
+                                    observer=clode.Observer.basic_all_variables, stepper=clode.Stepper.stochastic_euler,
+                                    **dict(common, dt=0.05, dtmax=0.05))
+        ou.set_ensemble(parameters={"theta": np.ones(n_each)})
+        sims = [vdp, lor, ck, ou]
+        for k, s in enumerate(sims):
+            s.set_member_tspan(0.0, 500.0 * np.exp(rng.standard_normal(n_each)))
+            s.seed_rng(100 + k)
+            s.enable_step_counter(True)
+        return sims
+
+    state = rng.bit_generator.state
+    sims = build()
+    batch = clode.BatchSimulator(sims)
+    t0 = time.perf_counter()
+    block, slc = batch.autotune(stride=16, max_steps=4096)
+    tune_s = time.perf_counter() - t0
+    for s in sims:
+        s.reset_step_counts()
+    t0 = time.perf_counter()
+    batch.sweep(rounds, graph=True)
+    graph_s = time.perf_counter() - t0
+    steps = sum(int(s.get_step_counts().sum()) for s in sims)
+    F_graph = [s.get_observer_results().to_ndarray() for s in sims]
+    # the same sequence, system by system, one-member-per-lane grid launches,
+    # over the first grid_rounds rounds (compared per round)
+    rng.bit_generator.state = state
+    sims2 = build()
+    grid_rounds = min(rounds, grid_rounds)
+    t0 = time.perf_counter()
+    for _ in range(grid_rounds):
+        for s in sims2:
+            s.transient()
+            s.features(fetch_results=False)
+    grid_s = time.perf_counter() - t0
+    same = None
+    if grid_rounds == rounds:
+        same = all(np.array_equal(a, b.get_observer_results().to_ndarray()) for a, b in zip(F_graph, sims2))
+    return {"members": 4 * n_each, "rounds": rounds, "autotuned": {"block": block, "slice": slc, "tune_s": tune_s},
+            "graph_sweep_s": graph_s, "s_per_round": graph_s / rounds, "member_steps": steps,
+            "member_steps_per_s": steps / graph_s, "grid_rounds": grid_rounds, "grid_s_per_round": grid_s / grid_rounds,
+            "speedup_vs_grid_per_round": (grid_s / grid_rounds) / (graph_s / rounds),
+            "identical_features": same}
+
+
+def surrogate_c5(precision, n=2048):
+    """2048 x 2048 lactotroph feature map (gK in [0, 10] x gBK in [0, 2],
+    dopri5 rtol 1e-6, transient 5e3 then thresh2 over 2e4): the plain map
+    against the surrogate-screened map (2^18 integrated training samples,
+    BF16-MFMA MLP, screening, re-solve of the flagged cells)."""
+    import clode_amd as clode
+    from clode_amd.surrogate import compare_maps, integrator_solver, screened_map
+
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lactotroph.cl"),
+                                 variables={"v": -60.0, "n": 0.0, "f": 0.0, "c": 0.1},
+                                 parameters={"gcal": 1.5, "gsk": 3.0, "gbk": 1.0, "gk": 2.0}, aux=["ica"],
+                                 observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                 single_precision=precision, dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6,
+                                 max_steps=10 ** 8, observer_x_up_thresh=0.3, observer_x_down_thresh=0.2)
+    solve = integrator_solver(sim, ("gk", "gbk"), ((0.0, 10.0), (0.0, 2.0)), 5000.0, 20000.0, "v")
+    solve(np.random.default_rng(0).random((4096, 2)))  # compile and first launch, untimed
+    gx, gy = np.meshgrid(np.arange(n) / (n - 1), np.arange(n) / (n - 1), indexing="ij")
+    t0 = time.perf_counter()
+    plain = solve(np.stack([gx.ravel(), gy.ravel()], axis=1))
+    plain_s = time.perf_counter() - t0
+    plain = {k: v.reshape(n, n) for k, v in plain.items()}
+    m, _ = screened_map(solve, n, n, seed=1)
+    bad = compare_maps({"period": m.period, "amp": m.amp, "cls": m.cls}, plain, tol=0.05)
+    return {"grid": f"{n}x{n}", "plain_map_s": plain_s, "screened_total_s": m.timings["total_s"],
+            "speedup_total": plain_s / m.timings["total_s"],
+            "speedup_screen_and_resolve": plain_s / (m.timings["screen_s"] + m.timings["resolve_s"]),
+            "timings": m.timings, "cells_integrated_frac": float(m.solved.mean()),
+            "cells_disagreeing_frac": float(bad.mean()), "disagreement_rule": "class differs, or period or "
+            "amplitude differs by more than 5 % relative to the plain map", "final_train_loss":
+            float(m.train_loss[-1]) if m.train_loss is not None else None}
+
+
+def profiled_traffic(kernel="clode_features"):
+    """Scaled HBM bytes per launch of the kernel, from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload."""
+    import re
+
+    path = os.path.join(REPO, "profiles", "features_lactotroph_256x256_dopri5_thresh2_fp64.md")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"^traffic_bytes_per_launch\s+%s\s+(\d+)" % kernel, text, re.M)
+    return int(m.group(1)) if m else None
 
 
 def main():
@@ -199,6 +399,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-members", type=int, default=1024)
+    ap.add_argument("--extras", default="fp32c,c1,c3,c4,c5",
+                    help="comma list of extra workloads: fp32c, c1, c3, c4, c5 (or 'none')")
+    ap.add_argument("--c4-rounds", type=int, default=20)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy kernel (FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -227,13 +430,23 @@ def main():
     feat_s = r64["kernel_ms"]["features"] / 1e3
     achieved = feat_steps * fl["flops"] / feat_s / 1e12
     roofline = {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
-                "frac": achieved / PEAK_TFLOPS["f64"], "traffic": None,
+                "frac": achieved / PEAK_TFLOPS["f64"], "traffic": profiled_traffic(),
                 "kernel": "clode_features (thresh2, dopri5, FP64)",
                 "flops_per_member_step": fl["flops"], "transcendentals_per_member_step": fl["transcendentals"],
                 "member_steps_per_launch": feat_steps, "launch_ms": r64["kernel_ms"]["features"]}
 
     extra = {}
-    if not args.no_extra:
+    wanted = set() if args.no_extra or args.extras == "none" else set(args.extras.split(","))
+
+    def run_extra(name, fn):
+        t0 = time.perf_counter()
+        try:
+            extra[name] = fn()
+        except Exception as e:  # an extra figure must not cost the headline line
+            extra[name] = {"error": f"{type(e).__name__}: {e}"}
+        print(f"# {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
+
+    if "fp32c" in wanted:
         r32 = run_lactotroph("fp32c", args.steps, args.warmup)
         print(f"# fp32c: {r32['member_steps_per_s']:.4e} member-steps/s, {r32['ms_per_step']:.1f} ms/step",
               file=sys.stderr, flush=True)
@@ -245,10 +458,15 @@ def main():
             "features_kernel_tflops": a32, "features_kernel_frac_of_fp32_peak": a32 / PEAK_TFLOPS["f32"],
             "feature_error_vs_fp64": feature_error(r32["features"], r64["features"], r64["feature_names"]),
         }
-        try:
-            extra["lorenz_2p20_dopri5_localmax"] = lorenz_ratio()
-        except Exception as e:  # extra figure only
-            extra["lorenz_2p20_dopri5_localmax"] = {"error": str(e)}
+    if "c1" in wanted:
+        run_extra("vdp_4096_rk4_fp64", vdp_c1)
+    if "c3" in wanted:
+        run_extra("lorenz_2p20_localmax", lorenz_c3)
+    if "c4" in wanted:
+        run_extra("mixed_2p20_persistent_sweep", lambda: mixed_c4(rounds=args.c4_rounds))
+    if "c5" in wanted:
+        run_extra("surrogate_2048_fp64", lambda: surrogate_c5("double"))
+        run_extra("surrogate_2048_fp32c", lambda: surrogate_c5("fp32c"))
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(args.cpu_members)

@@ -41,10 +41,11 @@ class BatchSimulator:
     def get_launch(self) -> Tuple[int, int]:
         return tuple(self._batch.get_launch())
 
-    def autotune(self, stride: int = 16) -> Tuple[int, int]:
+    def autotune(self, stride: int = 16, max_steps: int = 4096) -> Tuple[int, int]:
         """Pick the fastest (block size, slice) by timing the transient launch
-        over every stride-th queued member; member state is restored."""
-        return tuple(self._batch.autotune(stride))
+        over every stride-th queued member, each capped at max_steps steps
+        (0: no cap); member state is restored."""
+        return tuple(self._batch.autotune(stride, max_steps))
 
     def _invalidate(self, features: bool = False, x0: bool = False) -> None:
         for s in self._sims:

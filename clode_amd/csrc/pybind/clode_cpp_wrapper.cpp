@@ -564,12 +564,12 @@ public:
         py::gil_scoped_release nogil;
         check(clode_batch_sweep(b_, rounds, graph ? 1 : 0));
     }
-    py::tuple autotune(int stride)
+    py::tuple autotune(int stride, unsigned int max_steps)
     {
         int b = 0, s = 0;
         {
             py::gil_scoped_release nogil;
-            check(clode_batch_autotune(b_, stride, &b, &s));
+            check(clode_batch_autotune(b_, stride, max_steps, &b, &s));
         }
         return py::make_tuple(b, s);
     }
@@ -851,7 +851,7 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("features", &BatchSimulatorBasePy::features, py::arg("reinitialize") = -1)
         .def("shift_x0", &BatchSimulatorBasePy::shift_x0)
         .def("sweep", &BatchSimulatorBasePy::sweep, py::arg("rounds"), py::arg("graph") = true)
-        .def("autotune", &BatchSimulatorBasePy::autotune, py::arg("stride") = 16)
+        .def("autotune", &BatchSimulatorBasePy::autotune, py::arg("stride") = 16, py::arg("max_steps") = 4096)
         .def("last_kernel_ms", &BatchSimulatorBasePy::last_kernel_ms)
         .def("__len__", &BatchSimulatorBasePy::size);
 }

@@ -299,7 +299,7 @@ void Batch::sweep(int rounds, bool graph)
         s->observerInitialized = true;
 }
 
-void Batch::autotune(int stride, int &blockOut, int &sliceOut)
+void Batch::autotune(int stride, unsigned int maxSteps, int &blockOut, int &sliceOut)
 {
     if (stride < 1)
         throw InvalidArgument("autotune stride must be at least 1");
@@ -353,6 +353,8 @@ void Batch::autotune(int stride, int &blockOut, int &sliceOut)
                 for (size_t k = 0; k < a.size(); ++k) {
                     a[k].q_stride = (unsigned int)stride;
                     a[k].q_count = (unsigned int)((sims_[k]->nPts + stride - 1) / stride);
+                    if (maxSteps > 0)
+                        a[k].sp_max_steps = std::min(a[k].sp_max_steps, maxSteps);
                 }
                 if (!warmed) {  // first launch loads the module and warms the caches
                     enqueue("clode_batch_transient", a);

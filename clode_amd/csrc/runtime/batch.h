@@ -39,10 +39,11 @@ public:
     // whole sequence is captured into one graph and launched once
     void sweep(int rounds, bool graph);
 
-    // time the transient work-queue kernel on every stride-th queued member
-    // (state restored afterwards) for each candidate launch shape and keep the
-    // fastest; returns the chosen shape
-    void autotune(int stride, int &block, int &slice);
+    // time the transient work-queue kernel on every stride-th queued member,
+    // each trial capped at max_steps steps per member (0: no cap), for each
+    // candidate launch shape and keep the fastest (member state restored
+    // afterwards); returns the chosen shape
+    void autotune(int stride, unsigned int maxSteps, int &block, int &slice);
 
     // compile the batched program for these simulators (no device needed)
     static void compileOnly(const std::vector<Simulator *> &sims);

@@ -640,11 +640,11 @@ int clode_batch_sweep(clode_batch *batch, int rounds, int use_graph)
     CHECK_BATCH(batch) CLODE_TRY batch->b->sweep(rounds, use_graph != 0);
     CLODE_CATCH
 }
-int clode_batch_autotune(clode_batch *batch, int stride, int *block_size, int *slice)
+int clode_batch_autotune(clode_batch *batch, int stride, unsigned int max_steps, int *block_size, int *slice)
 {
     CHECK_BATCH(batch) CLODE_TRY
     int b = 0, s = 0;
-    batch->b->autotune(stride, b, s);
+    batch->b->autotune(stride, max_steps, b, s);
     if (block_size)
         *block_size = b;
     if (slice)

@@ -374,8 +374,14 @@ class Simulator:
             self._integrator.set_member_tspan([])
             return
         n = self._ensemble_size
-        t0 = np.broadcast_to(np.asarray(t_start, dtype=np.float64), self._ensemble_shape).flatten(order="F")
-        t1 = np.broadcast_to(np.asarray(t_end, dtype=np.float64), self._ensemble_shape).flatten(order="F")
+
+        def flat(t):
+            t = np.asarray(t, dtype=np.float64)
+            if t.size == n:
+                return t.flatten(order="F")
+            return np.broadcast_to(t, self._ensemble_shape).flatten(order="F")
+
+        t0, t1 = flat(t_start), flat(t_end)
         if t0.size != n or t1.size != n:
             raise ValueError("member time spans must broadcast to the ensemble shape")
         self._integrator.set_member_tspan(list(np.concatenate([t0, t1])))

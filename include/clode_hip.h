@@ -239,10 +239,12 @@ int clode_batch_shift_x0(clode_batch *batch);
 /* rounds x (transient, x0 <- xf, features, x0 <- xf); use_graph != 0 captures
  * the whole sequence into one graph and launches it once */
 int clode_batch_sweep(clode_batch *batch, int rounds, int use_graph);
-/* Time the transient launch over every stride-th queued member for each
- * block size in {64, 128, 256} and slice in {1, 4, 16, 64}, restoring the
- * member state after each trial, and keep the fastest. */
-int clode_batch_autotune(clode_batch *batch, int stride, int *block_size, int *slice);
+/* Time the transient launch over every stride-th queued member, each member
+ * capped at max_steps steps (0: no cap; a cap keeps a heavy tail of long
+ * members from dominating the trials), for each block size in {64, 128, 256}
+ * and slice in {1, 4, 16, 64}, restoring the member state after each trial,
+ * and keep the fastest. */
+int clode_batch_autotune(clode_batch *batch, int stride, unsigned int max_steps, int *block_size, int *slice);
 /* device time of the last batch operation, milliseconds */
 int clode_batch_last_ms(clode_batch *batch, double *ms);
 

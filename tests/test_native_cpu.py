@@ -29,6 +29,21 @@ def test_c_abi_exports_every_declared_symbol():
     assert not missing, missing
 
 
+def test_surrogate_abi_exports_every_declared_symbol():
+    text = open(os.path.join(REPO, "include", "clode_mlp.h")).read()
+    names = sorted(set(re.findall(r"^(?:int|const char \*)\s*(clode_mlp_\w+)\(", text, re.M)))
+    assert len(names) >= 12
+    lib = ctypes.CDLL(os.path.join(REPO, "clode_amd", "libclode_mlp.so"))
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+    # no device here: creation fails cleanly with a message
+    h = ctypes.c_void_p()
+    lib.clode_mlp_last_error.restype = ctypes.c_char_p
+    assert lib.clode_mlp_create(ctypes.byref(h), 0, ctypes.c_ulonglong(0)) == 2
+    assert b"device" in lib.clode_mlp_last_error()
+    assert lib.clode_mlp_destroy(None) == 0
+
+
 def test_c_abi_without_device():
     lib = ctypes.CDLL(LIB)
     n = ctypes.c_int(-1)
