@@ -36,6 +36,10 @@
 #ifndef CLODE_BLOCK_MAX
 #define CLODE_BLOCK_MAX 256
 #endif
+// extra kernel attributes (code-generation tuning), empty by default
+#ifndef CLODE_KERNEL_ATTRS
+#define CLODE_KERNEL_ATTRS
+#endif
 
 #if CLODE_PREC == 3
 typedef double xstore;   // x0 / xf: hi - c of the compensated pair
@@ -431,43 +435,43 @@ CLODE_DEV void run_queue(const ClodeArgs &a)
 }
 
 #ifndef CLODE_BATCH
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_transient(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_transient(const ClodeArgs a)
 {
     run_grid<TransientLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_trajectory(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_trajectory(const ClodeArgs a)
 {
     run_grid<TrajectoryLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_transient(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_queue_transient(const ClodeArgs a)
 {
     run_queue<TransientLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_trajectory(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_queue_trajectory(const ClodeArgs a)
 {
     run_queue<TrajectoryLane>(a);
 }
 
 #ifdef CLODE_OBSERVER
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_initialize_observer(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_initialize_observer(const ClodeArgs a)
 {
     run_grid<InitObsLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_features(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_features(const ClodeArgs a)
 {
     run_grid<FeaturesLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_initialize_observer(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_queue_initialize_observer(const ClodeArgs a)
 {
     run_queue<InitObsLane>(a);
 }
 
-extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) clode_queue_features(const ClodeArgs a)
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_queue_features(const ClodeArgs a)
 {
     run_queue<FeaturesLane>(a);
 }

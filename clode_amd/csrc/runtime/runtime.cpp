@@ -14,6 +14,7 @@
 #include <cstring>
 #include <fstream>
 #include <iostream>
+#include <cstdlib>
 #include <sstream>
 #include <sys/stat.h>
 #include <unistd.h>
@@ -357,6 +358,14 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
         o.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
     else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
+    // code-generation tuning: CLODE_HIPRTC_OPTIONS holds extra whitespace-
+    // separated compiler options (part of the program cache key)
+    if (const char *extra = std::getenv("CLODE_HIPRTC_OPTIONS")) {
+        std::istringstream in(extra);
+        std::string tok;
+        while (in >> tok)
+            o.push_back(tok);
+    }
     return o;
 }
 
