@@ -402,6 +402,9 @@ def main():
     ap.add_argument("--extras", default="fp32c,c1,c3,c4,c5",
                     help="comma list of extra workloads: fp32c, c1, c3, c4, c5 (or 'none')")
     ap.add_argument("--c4-rounds", type=int, default=20)
+    ap.add_argument("--mlp-kernels", action="store_true",
+                    help="run only the surrogate kernels (one training epoch over 65,536 synthetic samples and one "
+                         "2048 x 2048 screening pass), for profiling")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy kernel (FETCH_SIZE/WRITE_SIZE calibration)")
     args = ap.parse_args()
@@ -414,6 +417,20 @@ def main():
             torch.mul(x, 1.0, out=y)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes_read": x.numel() * 4, "copy_bytes_written": y.numel() * 4, "launches": 3}))
+        return
+    if args.mlp_kernels:
+        from clode_amd.surrogate import MLPSurrogate
+
+        rng = np.random.default_rng(0)
+        U = rng.random((65536, 2))
+        T = np.stack([np.sin(3 * U[:, 0]), U[:, 0] * U[:, 1]], axis=1)
+        cls = np.clip((8 * U[:, 0]).astype(int), 0, 15)
+        net = MLPSurrogate(seed=1)
+        hist = net.train(U, T, cls, epochs=1, batch=2048)
+        train_ms = net.last_ms()
+        net.screen_grid(2048, 2048)
+        print(json.dumps({"train_epoch_ms": train_ms, "train_loss": float(hist[-1]), "screen_ms": net.last_ms(),
+                          "screen_tflops": 4194304 * 2 * 238720 / (net.last_ms() / 1e3) / 1e12}))
         return
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
