@@ -14,6 +14,8 @@ from __future__ import annotations
 from .cpp.clode_cpp_wrapper import ObserverParams, ProblemInfo, SolverParams
 from .batch import BatchSimulator
 from .features import FeatureSimulator, Observer, ObserverOutput
+from .function_converter import OpenCLConverter, OpenCLRhsEquation, convert_str_to_opencl
+from .opencl_builtins import *  # noqa: F401,F403  (the built-ins a Python right-hand side may call)
 from .runtime import (
     CLDeviceType,
     CLVendor,
@@ -29,10 +31,15 @@ from .runtime import (
 )
 from .solver import Simulator, Stepper
 from .trajectory import TrajectoryOutput, TrajectorySimulator
+from .xpp_parser import convert_xpp_file
 
 __version__ = "0.1.0"
 
 __all__ = [
+    "OpenCLConverter",
+    "OpenCLRhsEquation",
+    "convert_str_to_opencl",
+    "convert_xpp_file",
     "BatchSimulator",
     "CLDeviceType",
     "CLVendor",

@@ -8,6 +8,7 @@ flattening at the boundary and lazy device-to-host transfers.
 """
 from __future__ import annotations
 
+import os
 from enum import Enum
 from typing import Any, Callable, Dict, List, Mapping, Optional, Tuple, Union
 
@@ -43,7 +44,13 @@ def _write_rhs_source(rhs_equation, supplementary_equations) -> str:
         for eq in supplementary_equations:
             converter.convert_to_opencl(eq)
     text = converter.convert_to_opencl(rhs_equation, mutable_args=[3, 4], function_name="getRHS")
-    path = "clode_rhs.cl"
+    # the program builder reads the right-hand side from a file; name it by
+    # content so that equal sources share a file (and a compiled program)
+    import hashlib
+    import tempfile
+
+    digest = hashlib.sha1(text.encode()).hexdigest()[:16]
+    path = os.path.join(tempfile.gettempdir(), f"clode_rhs_{digest}.cl")
     with open(path, "w") as f:
         f.write(text)
     return path
