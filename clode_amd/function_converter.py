@@ -398,8 +398,11 @@ class _Function:
             suffix = "[]" if t.endswith("[]") else ""
             args.append(f"{'const ' if const else ''}{base} {name}{suffix}")
         self.body(self.node.body, 1)
-        head = f"{self.ret} {self.name}(" + ", ".join(args) + ") {"
-        return "\n".join([head] + self.lines + ["}"]) + "\n"
+        # one parameter per line, aligned under the first (the layout the
+        # reference's converter tests expect, test_function_converter.py:85-181)
+        lead = f"{self.ret} {self.name}("
+        head = lead + (",\n" + " " * len(lead)).join(args) + ") {"
+        return "\n".join([head] + self.lines + ["}"]) + "\n\n"
 
 
 class OpenCLConverter:
@@ -417,7 +420,7 @@ class OpenCLConverter:
         text = fn.translate()
         self._functions.append(text)
         self._known[fn.name] = fn.ret
-        return "\n".join(self._functions)
+        return "".join(self._functions)
 
     def convert_to_opencl(self, function: Callable, mutable_args: Optional[Sequence[int]] = None,
                           function_name: Optional[str] = None) -> str:

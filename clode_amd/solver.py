@@ -90,7 +90,9 @@ class Simulator:
         device_id: Optional[int] = None,
         device_ids: Optional[List[int]] = None,
     ) -> None:
+        self._xpp_outputs: Optional[Tuple[List[str], List[str]]] = None
         input_file = self._handle_clode_rhs_cl_file(src_file, rhs_equation, supplementary_equations)
+        aux, num_noise = self._check_xpp_outputs(aux, num_noise)
         self._pi = ProblemInfo(input_file, list(variables.keys()), list(parameters.keys()),
                                list(aux) if aux is not None else [], num_noise)
         self._stepper = stepper
@@ -160,13 +162,35 @@ class Simulator:
             raise ValueError("Cannot specify both src_file and rhs_equation")
         if src_file is not None:
             if src_file.endswith(".xpp") or src_file.endswith(".ode"):
-                from .xpp_parser import convert_xpp_file
+                from .xpp_parser import convert_xpp_file, read_ode_parameters
 
+                with open(src_file) as f:
+                    parsed = read_ode_parameters(f.read())
+                self._xpp_outputs = (list(parsed[1]), list(parsed[4]))
                 return convert_xpp_file(src_file)
             return src_file
         if rhs_equation is not None:
             return _write_rhs_source(rhs_equation, supplementary_equations)
         raise ValueError("Must specify either src_file or rhs_equation")
+
+    def _check_xpp_outputs(self, aux: Optional[List[str]], num_noise: int) -> Tuple[Optional[List[str]], int]:
+        """An XPP file fixes how many aux_[] entries getRHS writes and how many
+        w_[] entries it reads: default the aux names and noise count to the
+        file's, and refuse smaller ones (the kernel would index past its
+        arrays)."""
+        if self._xpp_outputs is None:
+            return aux, num_noise
+        file_aux, file_noise = self._xpp_outputs
+        if aux is None:
+            aux = file_aux
+        elif len(aux) < len(file_aux):
+            raise ValueError(f"The XPP file defines {len(file_aux)} auxiliary outputs ({', '.join(file_aux)}); "
+                             f"aux names {len(aux)}")
+        if num_noise == 0:
+            num_noise = len(file_noise)
+        elif num_noise < len(file_noise):
+            raise ValueError(f"The XPP file uses {len(file_noise)} noise terms; num_noise is {num_noise}")
+        return aux, num_noise
 
     # ---------------------------------------------------------------- ensembles
     def set_repeat_ensemble(self, num_repeats: int) -> None:

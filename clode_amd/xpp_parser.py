@@ -1,128 +1,146 @@
-"""XPP/XPPAUT ``.ode`` files -> OpenCL-C right-hand sides.
+"""XPP/XPPAUT ``.ode``/``.xpp`` files -> OpenCL-C right-hand sides.
 
-Counterpart of clode/xpp_parser.py.  Understood lines:
+Counterpart of clode/xpp_parser.py.  The generated text has the layout the
+reference produces and tests against a golden file
+(test/test_xpp_parser.py:9-22): a ``getRHS`` with one parameter per line and
+eight commented sections (state variables, parameters, noise terms, core
+equations, auxiliary equations, differential equations, auxiliary outputs,
+differential outputs).  Expressions keep their XPP text except for:
 
-  ``par a=1, b=2`` / ``p ...``    parameters (in order: p_[0], p_[1], ...)
-  ``init x=0.1, y=0``              state variables and initial values (x_[0], ...)
+- powers ``a^k``: k = 2, 3, 4 become repeated products, other unsigned
+  integers ``pown(a, k)``, signed or fractional exponents ``pow(a, k)``;
+- decimal literals get an ``f`` suffix (so, as in the reference, they are
+  single-precision constants even when ``realtype`` is double);
+- ``%`` starts a comment that runs to the end of the line.
+
+Lines understood (at the start of the line, as XPP writes them):
+
+  ``par a=1, b=2`` / ``p ...``     parameters (p_[0], p_[1], ... in order)
+  ``init x=0.1, y=0``              state variables (x_[0], ... in order)
   ``x' = expr`` / ``dx/dt = expr`` differential equations
   ``aux name = expr``              auxiliary outputs (aux_[0], ...)
-  ``wiener w``                     noise terms (w_[0], ...)
-  ``name = expr``                  intermediate quantities, in file order
-  ``% ...``                        comments, kept as C comments
-  ``# ...``, ``@ ...``, ``done``   ignored
+  ``wiener w1 w2``                 noise terms (w_[0], ...)
+  ``@ ...``, blank lines           ignored
+  ``# ...``, ``done``              ignored (XPP comments and terminator)
+  anything else                    copied as a statement (``name = expr``
+                                   becomes a ``realtype`` declaration)
 
-Expressions use XPP syntax; ``^`` and ``**`` powers become multiplication
-(exponents 2..4), ``pown`` (other integers) or ``pow``, and float literals
-become ``RCONST`` literals, through the expression translator of
-clode_amd/function_converter.py.  State variables that only appear in
-``x' =`` lines (without ``init``) are added in order of appearance.
+A state variable without an equation, or an equation for a variable with no
+``init`` line, raises ``ValueError`` (the reference emits code that then
+fails to compile).
 """
 from __future__ import annotations
 
-import ast
 import re
 from typing import Dict, List, Tuple
 
-from .function_converter import REAL, REAL_ARRAY, _Function
+_DEF_NAME = re.compile(r"(\w+)\s*=(?!=)\s*")
+_DX_LINE = re.compile(r"^(\w+)'\s*=\s*")
+_DXDT_LINE = re.compile(r"^d(\w+)/dt\s*=\s*")
+_PREFIXES = {"par": "par", "p": "par", "param": "par", "init": "init", "i": "init", "aux": "aux",
+             "wiener": "wiener"}
 
-_DEF = re.compile(r"(\w+)\s*=\s*")
+_POWER = re.compile(r"(\w+)\s*\^\s*([-+]?[\d.]*\d)")
+_DECIMAL = re.compile(r"(\d+\.\d*(?:[eE][-+]?\d+)?)")
 
 
-def _definitions(body: str) -> List[Tuple[str, str]]:
-    """'a=1, b = 2*c, c=3' -> [(a, 1), (b, 2*c), (c, 3)]; a comma separates
-    definitions only where a new 'name=' follows."""
-    out: List[Tuple[str, str]] = []
-    matches = [m for m in _DEF.finditer(body) if body[m.end():m.end() + 1] != "="
-               and (m.start() == 0 or body[:m.start()].rstrip().endswith(",") or not body[:m.start()].strip())]
-    for k, m in enumerate(matches):
-        end = matches[k + 1].start() if k + 1 < len(matches) else len(body)
-        value = body[m.end():end].strip().rstrip(",").strip()
-        out.append((m.group(1), value))
-    if not out and body.strip():
-        raise ValueError(f"Could not parse definitions '{body.strip()}'")
+def _definitions(body: str, line: str) -> List[Tuple[str, str]]:
+    """``'a=1, b = 2*c, c=3'`` -> ``[('a', '1'), ('b', '2*c'), ('c', '3')]``.
+    A comma ends a value only where another ``name=`` follows it."""
+    starts = [m for m in _DEF_NAME.finditer(body)
+              if not body[:m.start()].strip() or body[:m.start()].rstrip().endswith(",")]
+    if not starts:
+        raise ValueError(f"Could not parse line '{line}'")
+    out = []
+    for k, m in enumerate(starts):
+        end = starts[k + 1].start() if k + 1 < len(starts) else len(body)
+        out.append((m.group(1), body[m.end():end].strip().rstrip(",").strip()))
     return out
 
 
-def read_ode_parameters(xpp_string: str):
+def read_ode_parameters(
+    xpp_string: str,
+) -> Tuple[Dict[str, str], Dict[str, str], Dict[str, str], Dict[str, str], List[str], List[str]]:
     """(parameters, auxiliaries, initial_values, dx, noise, statements) of an
-    XPP file: dicts name -> expression text (in file order), the noise names,
-    and the intermediate statements / comments as (name, expr) or ('%', text)."""
+    XPP file, as in clode/xpp_parser.py:38-75: dicts name -> expression text in
+    file order, the noise names, and the remaining lines as statements."""
     parameters: Dict[str, str] = {}
     auxiliaries: Dict[str, str] = {}
     initial_values: Dict[str, str] = {}
     dx: Dict[str, str] = {}
     noise: List[str] = []
-    statements: List[Tuple[str, str]] = []
-    for raw in xpp_string.splitlines():
-        line = raw.strip()
-        if not line or line.startswith("#") or line.startswith("@") or line.lower() == "done":
-            continue
-        if line.startswith("%"):
-            statements.append(("%", line[1:].strip()))
-            continue
+    statements: List[str] = []
+    stores = {"par": parameters, "init": initial_values, "aux": auxiliaries}
+    for line in xpp_string.splitlines():
         head, _, rest = line.partition(" ")
-        if head in ("par", "p", "param"):
-            parameters.update(_definitions(rest))
-        elif head in ("init", "i"):
-            initial_values.update(_definitions(rest))
-        elif head == "aux":
-            auxiliaries.update(_definitions(rest))
-        elif head == "wiener":
-            noise.extend(n.strip() for n in rest.replace(",", " ").split() if n.strip())
+        kind = _PREFIXES.get(head) if rest else None
+        if kind == "wiener":
+            noise.extend(rest.replace(",", " ").split())
+        elif kind is not None:
+            stores[kind].update(_definitions(rest, line))
+        elif _DX_LINE.match(line) or _DXDT_LINE.match(line):
+            m = _DX_LINE.match(line) or _DXDT_LINE.match(line)
+            dx[m.group(1)] = line[m.end():]
+        elif not line.strip() or line.startswith("@") or line.startswith("#") or line.strip() == "done":
+            continue
         else:
-            m = re.match(r"^(\w+)'\s*=\s*(.+)$", line) or re.match(r"^d(\w+)/dt\s*=\s*(.+)$", line)
-            if m:
-                dx[m.group(1)] = m.group(2).strip()
-                continue
-            m = re.match(r"^(\w+)\s*=\s*(.+)$", line)
-            if m:
-                statements.append((m.group(1), m.group(2).strip()))
-                continue
-            raise ValueError(f"Could not parse line '{raw}'")
+            statements.append(line)
     return parameters, auxiliaries, initial_values, dx, noise, statements
 
 
-def _translate(expr: str, fn: _Function) -> str:
-    py = expr.replace("^", "**")
-    return fn.expr(ast.parse(py, mode="eval").body).text
+def _power(m: re.Match) -> str:
+    base, exponent = m.group(1), m.group(2)
+    if exponent in ("2", "3", "4"):
+        return "*".join([base] * int(exponent))
+    if exponent.isdigit():
+        return f"pown({base}, {exponent})"
+    return f"pow({base}, {exponent})"
+
+
+def _rewrite_line(line: str) -> str:
+    line = _DECIMAL.sub(r"\1f", _POWER.sub(_power, line))
+    cut = line.find("%")
+    return line if cut < 0 else f"{line[:cut]}/* {line[cut + 1:]} */"
 
 
 def format_opencl_rhs(parameters, auxiliaries, initial_values, dx, noise, statements) -> str:
-    variables = list(initial_values) + [v for v in dx if v not in initial_values]
+    """OpenCL-C ``getRHS`` text for the parsed file (layout of
+    clode/xpp_parser.py:78-177)."""
+    variables = list(initial_values)
     missing = [v for v in variables if v not in dx]
     if missing:
         raise ValueError(f"No differential equation for {', '.join(missing)}")
-    stub = ast.parse("def getRHS(t: float) -> None:\n    pass\n").body[0]
-    fn = _Function(stub, {}, (), "getRHS")
-    scope = fn.scopes[0]
-    lines = ["void getRHS(const realtype t, const realtype x_[], const realtype p_[],",
-             "            realtype dx_[], realtype aux_[], const realtype w_[])", "{"]
-    for k, v in enumerate(variables):
-        lines.append(f"    realtype {v} = x_[{k}];")
-        scope[v] = REAL
-    for k, p in enumerate(parameters):
-        lines.append(f"    realtype {p} = p_[{k}];")
-        scope[p] = REAL
-    for k, w in enumerate(noise):
-        lines.append(f"    realtype {w} = w_[{k}];")
-        scope[w] = REAL
-    for name, value in statements:
-        if name == "%":
-            lines.append(f"    /* {value} */")
-            continue
-        lines.append(f"    realtype {name} = {_translate(value, fn)};")
-        scope[name] = REAL
-    for name, value in auxiliaries.items():
-        lines.append(f"    realtype {name} = {_translate(value, fn)};")
-        scope[name] = REAL
-    for v in variables:
-        lines.append(f"    realtype d_{v} = {_translate(dx[v], fn)};")
-    for k, name in enumerate(auxiliaries):
-        lines.append(f"    aux_[{k}] = {name};")
-    for k, v in enumerate(variables):
-        lines.append(f"    dx_[{k}] = d_{v};")
-    lines.append("}")
-    return "\n".join(lines) + "\n"
+    unknown = [v for v in dx if v not in initial_values]
+    if unknown:
+        raise ValueError(f"No init line for {', '.join(unknown)}")
+
+    def declare(line: str) -> str:
+        if "=" not in line:
+            return f"    {line};"
+        indent = len(line) - len(line.lstrip())
+        return f"    {line[:indent]}realtype {line[indent:]};"
+
+    sections = [
+        ("State variables", [f"    realtype {v} = x_[{k}];" for k, v in enumerate(variables)]),
+        ("Parameters", [f"    realtype {p} = p_[{k}];" for k, p in enumerate(parameters)]),
+        ("Noise terms", [f"    realtype {w} = w_[{k}];" for k, w in enumerate(noise)]),
+        ("Core equations", [declare(s) for s in statements]),
+        ("Auxiliary equations", [f"    realtype {a} = {e};" for a, e in auxiliaries.items()]),
+        ("Differential equations", [f"    realtype d{v} = {e};" for v, e in dx.items()]),
+        ("Auxiliary outputs", [f"    aux_[{k}] = {a};" for k, a in enumerate(auxiliaries)]),
+        ("Differential outputs", [f"    dx_[{k}] = d{v};" for k, v in enumerate(variables)]),
+    ]
+    params = ["const realtype t", "const realtype x_[]", "const realtype p_[]", "realtype dx_[]",
+              "realtype aux_[]", "const realtype w_[]"]
+    lead = "void getRHS("
+    out = [lead + (",\n" + " " * len(lead)).join(params) + ") {", ""]
+    for k, (title, body) in enumerate(sections):
+        out.append(f"    /* {title} */")
+        out.extend(body)
+        if k + 1 < len(sections):
+            out.append("")
+    out.append("}")
+    return "\n".join(_rewrite_line(line) for line in "\n".join(out).split("\n"))
 
 
 def convert_xpp_file(filename: str) -> str:

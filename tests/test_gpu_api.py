@@ -4,6 +4,7 @@ test/test_ornl_thompson_a1.py), plus ensemble construction semantics of
 clode/solver.py:254-462."""
 import math
 import os
+from typing import List
 
 import numpy as np
 import pytest
@@ -13,6 +14,7 @@ import clode_amd as clode
 pytestmark = pytest.mark.gpu
 
 MODELS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "clode_amd", "models")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
 def vdp_period(mu):
@@ -23,9 +25,32 @@ def vdp_period(mu):
     return min(2 * math.pi * (1 + mu ** 2 / 16), (3 - 2 * math.log(2)) * mu + 3 * 2.2338 / mu ** (1 / 3))
 
 
+def vdp_rhs(t: float, var: List[float], par: List[float], derivatives: List[float], aux: List[float],
+            wiener: List[float]) -> None:
+    mu: float = par[0]
+    x: float = var[0]
+    y: float = var[1]
+    dx: float = y
+    dy: float = mu * (1 - x * x) * y - x
+    derivatives[0] = dx
+    derivatives[1] = dy
+
+
+@pytest.mark.parametrize("source", ["cl", "xpp", "python"])
 @pytest.mark.parametrize("precision", [True, False, "fp32c"])
-def test_vdp_dormand_prince(gpu_available, precision):
-    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"), variables={"x": 1.0, "y": 1.0},
+def test_vdp_dormand_prince(gpu_available, precision, source, tmp_path):
+    """test/test_vdp.py:47-102 with the model file, the reference's XPP test
+    file (test/test_xpp_parser.py:19) and a Python right-hand side."""
+    if source == "cl":
+        kw = dict(src_file=os.path.join(MODELS, "van_der_pol.cl"))
+    elif source == "xpp":
+        xpp = tmp_path / "van_der_pol_oscillator.xpp"
+        with open(os.path.join(GOLDEN, "xpp", "van_der_pol_oscillator.xpp")) as f:
+            xpp.write_text(f.read())
+        kw = dict(src_file=str(xpp))
+    else:
+        kw = dict(rhs_equation=vdp_rhs)
+    sim = clode.FeatureSimulator(**kw, variables={"x": 1.0, "y": 1.0},
                                  parameters={"mu": 1.0}, observer=clode.Observer.threshold_2,
                                  stepper=clode.Stepper.dormand_prince, t_span=(0.0, 1000.0), max_store=20000,
                                  max_steps=20000, single_precision=precision)
