@@ -38,17 +38,15 @@ namespace cm {
 // ---------------------------------------------------------------- double --
 CM_FN double exp(double x)
 {
-    if (!(x == x))
-        return x;
-    if (x > 709.782712893384)
-        return __builtin_huge_val();
-    if (x < -745.1332191019412)
-        return 0.0;
+    // Branch-free: the reduction runs on x clamped to the finite-result range
+    // (NaN clamps to -746) and the IEEE special cases are selected at the end,
+    // so the compiler can interleave independent calls.
     const double log2e = 0x1.71547652b82fep+0;
     const double ln2hi = 0x1.62e42fefa39efp-1;
     const double ln2lo = 0x1.abc9e3b39803fp-56;
-    double k = rint(x * log2e);
-    double r = fma(-k, ln2hi, x);
+    double xc = fmin(fmax(x, -746.0), 710.0);
+    double k = rint(xc * log2e);
+    double r = fma(-k, ln2hi, xc);
     r = fma(-k, ln2lo, r);
     double p = 0x1.6124613a86d09p-33;
     p = fma(p, r, 0x1.1eed8eff8d898p-29);
@@ -64,7 +62,10 @@ CM_FN double exp(double x)
     p = fma(p, r, 0x1.0000000000000p-1);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
-    return ldexp(p, (int)k);
+    double y = ldexp(p, (int)k);
+    y = (x > 709.782712893384) ? __builtin_huge_val() : y;
+    y = (x < -745.1332191019412) ? 0.0 : y;
+    return (x == x) ? y : x;
 }
 
 CM_FN double log(double x)
@@ -153,40 +154,34 @@ CM_FN double log_dd(double x, double *lo)
 
 CM_FN double pow(double x, double y)
 {
-    if (y == 0.0 || x == 1.0)
-        return 1.0;
-    if (!(x == x) || !(y == y))
-        return x + y;  // NaN
+    // Branch-free: the double-double path runs on a sanitised |x| and the
+    // IEEE-754 special cases are selected afterwards, highest priority last.
     double ax = fabs(x);
     double ay = fabs(y);
     bool yint = (y == rint(y));
     bool yodd = yint && ay < 0x1.0p53 && (0.5 * y != rint(0.5 * y));
-    if (ay == __builtin_huge_val()) {
-        if (ax == 1.0)
-            return 1.0;
-        return ((ax > 1.0) == (y > 0.0)) ? __builtin_huge_val() : 0.0;
-    }
-    if (x < 0.0 && !yint)
-        return __builtin_nan("");
-    if (ax == 0.0) {
-        if (y > 0.0)
-            return yodd ? x : 0.0;
-        return yodd ? __builtin_copysign(__builtin_huge_val(), x) : __builtin_huge_val();
-    }
-    if (ax == __builtin_huge_val()) {
-        double r = (y > 0.0) ? __builtin_huge_val() : 0.0;
-        return (x < 0.0 && yodd) ? -r : r;
-    }
+    bool neg = x < 0.0 && yodd;
+    double axs = (ax > 0.0 && ax < __builtin_huge_val()) ? ax : 1.0;
+    double ys = (ay < __builtin_huge_val()) ? y : 0.0;  // also NaN -> 0
     // y log|x| in double-double, then exp(p + q) = exp(p) (1 + q)
     double llo;
-    double lhi = cm::log_dd(ax, &llo);
-    double p = y * lhi;
-    double q = fma(y, lhi, -p);
-    q = fma(y, llo, q);
+    double lhi = cm::log_dd(axs, &llo);
+    double p = ys * lhi;
+    double q = fma(ys, lhi, -p);
+    q = fma(ys, llo, q);
     double r = cm::exp(p);
-    if (r != 0.0 && r != __builtin_huge_val())
-        r = fma(r, q, r);
-    return (x < 0.0 && yodd) ? -r : r;
+    r = (r != 0.0 && r != __builtin_huge_val()) ? fma(r, q, r) : r;
+    r = neg ? -r : r;
+    const double inf = __builtin_huge_val();
+    double rinf = (y > 0.0) ? inf : 0.0;                 // |x| = inf
+    r = (ax == inf) ? (neg ? -rinf : rinf) : r;
+    double rzero = (y > 0.0) ? (yodd ? x : 0.0) : (yodd ? __builtin_copysign(inf, x) : inf);
+    r = (ax == 0.0) ? rzero : r;                         // x = +-0
+    r = (x < 0.0 && !yint) ? __builtin_nan("") : r;
+    double ryinf = (ax == 1.0) ? 1.0 : (((ax > 1.0) == (y > 0.0)) ? inf : 0.0);
+    r = (ay == inf) ? ryinf : r;                         // y = +-inf
+    r = (!(x == x) || !(y == y)) ? x + y : r;           // NaN
+    return (y == 0.0 || x == 1.0) ? 1.0 : r;
 }
 
 CM_FN double pown(double x, int n)
@@ -206,17 +201,13 @@ CM_FN double pown(double x, int n)
 // ----------------------------------------------------------------- float --
 CM_FN float exp(float x)
 {
-    if (!(x == x))
-        return x;
-    if (x > 88.72283935546875f)
-        return __builtin_huge_valf();
-    if (x < -103.97208404541015625f)
-        return 0.0f;
+    // branch-free, as the double version
     const float log2e = 0x1.715476p+0f;
     const float ln2hi = 0x1.62e430p-1f;
     const float ln2lo = -0x1.05c610p-29f;
-    float k = rintf(x * log2e);
-    float r = fmaf(-k, ln2hi, x);
+    float xc = fminf(fmaxf(x, -104.0f), 89.0f);
+    float k = rintf(xc * log2e);
+    float r = fmaf(-k, ln2hi, xc);
     r = fmaf(-k, ln2lo, r);
     float p = 0x1.a01a02p-13f;
     p = fmaf(p, r, 0x1.6c16c2p-10f);
@@ -226,7 +217,10 @@ CM_FN float exp(float x)
     p = fmaf(p, r, 0x1.000000p-1f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
-    return ldexpf(p, (int)k);
+    float y = ldexpf(p, (int)k);
+    y = (x > 88.72283935546875f) ? __builtin_huge_valf() : y;
+    y = (x < -103.97208404541015625f) ? 0.0f : y;
+    return (x == x) ? y : x;
 }
 
 CM_FN float log(float x)
