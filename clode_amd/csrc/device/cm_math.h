@@ -70,19 +70,14 @@ CM_FN double exp(double x)
 
 CM_FN double log(double x)
 {
-    if (!(x > 0.0)) {
-        if (x == 0.0)
-            return -__builtin_huge_val();
-        return __builtin_nan("");  // negative or NaN
-    }
-    if (x == __builtin_huge_val())
-        return x;
+    // Branch-free: the reduction runs on x, or on 1 where x is not a positive
+    // finite number, and the IEEE special cases are selected at the end.
+    double xs = (x > 0.0 && x < __builtin_huge_val()) ? x : 1.0;
     int e;
-    double m = frexp(x, &e);  // x = m 2^e, m in [0.5, 1)
-    if (m < 0x1.6a09e667f3bcdp-1) {
-        m = m * 2.0;
-        e = e - 1;
-    }
+    double m = frexp(xs, &e);  // xs = m 2^e, m in [0.5, 1)
+    bool lowm = m < 0x1.6a09e667f3bcdp-1;
+    m = lowm ? m * 2.0 : m;
+    e = lowm ? e - 1 : e;
     const double ln2hi = 0x1.62e42fee00000p-1;  // 21 trailing zero bits: e*ln2hi exact
     const double ln2lo = 0x1.a39ef35793c76p-33;
     double f = m - 1.0;
@@ -103,7 +98,10 @@ CM_FN double log(double x)
     double hfsq = (0.5 * f) * f;
     double de = (double)e;
     double lo = fma(s, hfsq + R, de * ln2lo);
-    return de * ln2hi - ((hfsq - lo) - f);
+    double y = de * ln2hi - ((hfsq - lo) - f);
+    y = (x == __builtin_huge_val()) ? x : y;
+    y = (x == 0.0) ? -__builtin_huge_val() : y;
+    return (x < 0.0 || !(x == x)) ? __builtin_nan("") : y;
 }
 
 // log(x) as an unevaluated sum hi + lo (|lo| <= ulp(hi)), x finite and > 0:
@@ -225,19 +223,13 @@ CM_FN float exp(float x)
 
 CM_FN float log(float x)
 {
-    if (!(x > 0.0f)) {
-        if (x == 0.0f)
-            return -__builtin_huge_valf();
-        return __builtin_nanf("");
-    }
-    if (x == __builtin_huge_valf())
-        return x;
+    // branch-free, as the double version
+    float xs = (x > 0.0f && x < __builtin_huge_valf()) ? x : 1.0f;
     int e;
-    float m = frexpf(x, &e);
-    if (m < 0x1.6a09e6p-1f) {
-        m = m * 2.0f;
-        e = e - 1;
-    }
+    float m = frexpf(xs, &e);
+    bool lowm = m < 0x1.6a09e6p-1f;
+    m = lowm ? m * 2.0f : m;
+    e = lowm ? e - 1 : e;
     const float ln2hi = 0x1.62e400p-1f;  // 15 trailing zero bits: e*ln2hi exact
     const float ln2lo = 0x1.7f7d1cp-20f;
     float f = m - 1.0f;
@@ -251,7 +243,10 @@ CM_FN float log(float x)
     float hfsq = (0.5f * f) * f;
     float de = (float)e;
     float lo = fmaf(s, hfsq + R, de * ln2lo);
-    return de * ln2hi - ((hfsq - lo) - f);
+    float y = de * ln2hi - ((hfsq - lo) - f);
+    y = (x == __builtin_huge_valf()) ? x : y;
+    y = (x == 0.0f) ? -__builtin_huge_valf() : y;
+    return (x < 0.0f || !(x == x)) ? __builtin_nanf("") : y;
 }
 
 CM_FN float pow(float x, float y)
