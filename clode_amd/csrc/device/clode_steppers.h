@@ -60,6 +60,33 @@ struct SolverParamsR {
     unsigned int nout;
 };
 
+// The drivers call step_attempt() once per loop iteration.  A fixed-step
+// stepper completes its step in one attempt.  The adaptive controller makes
+// one trial per attempt and keeps what its retry loop carries between trials
+// (adaptive_explicit_step.clh:14-62: the trial step size, the attempt count
+// and whether a trial has failed) in StepCtl, so a lane whose trial was
+// rejected retries on the next iteration while the other lanes of its wave go
+// on with their next steps.  Each member still sees exactly the reference's
+// sequence of trials; only the interleaving across lanes changes.
+#if CLODE_PREC == 3
+typedef double ctlreal;
+#else
+typedef realtype ctlreal;
+#endif
+struct StepCtl {
+    ctlreal newDt;
+    int attempts;
+    bool noFailedSteps;
+    bool inStep;
+};
+CLODE_DEV void step_ctl_reset(StepCtl &c)
+{
+    c.newDt = 0;
+    c.attempts = 0;
+    c.noFailedSteps = true;
+    c.inStep = false;
+}
+
 #if CLODE_PREC != 3
 // =====================================================================
 //  Plain single / double precision (reference semantics)
@@ -116,9 +143,9 @@ CLODE_DEV void do_step(realtype &ti, realtype xi[], realtype k1[], const realtyp
 #if CLODE_FIXED_STEP
 // Fixed-step wrapper: dt never changes; k1 is re-evaluated at the new point
 // (after a fresh Wiener draw for seuler) and reused as the next step's k1.
-CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
-                      const SolverParamsR &sp, realtype &dt, const realtype t1,
-                      realtype aux[], realtype wi[], RngData &rd)
+CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                            const SolverParamsR &sp, realtype &dt, const realtype t1,
+                            realtype aux[], realtype wi[], RngData &rd, StepCtl &)
 {
     do_step(ti, xi, k1, pars, dt, aux, wi);
 #if CLODE_STOCHASTIC
@@ -126,7 +153,7 @@ CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype
         wi[j] = rng_normal(rd) / sqrt(dt);
 #endif
     getRHS(ti, xi, pars, k1, aux, wi);
-    return 0;
+    return true;
 }
 #endif
 
@@ -238,54 +265,57 @@ CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const rea
 // first rejection shrinks by max(MAX_SHRINK, 0.8 (rtol/err)^(1/(p+1))),
 // later rejections halve, growth after an unrejected step is capped at
 // MAX_GROW, and the step is clipped to the end of the span and to
-// [hmin, dtmax] with hmin = 16 ulp(t).  Returns -1 (state unchanged, dt=hmin)
-// when a step at hmin still fails; the drivers ignore the flag, as the
-// reference's do.
-CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
-                      const SolverParamsR &sp, realtype &dt, const realtype t1,
-                      realtype aux[], realtype wi[], RngData &rd)
+// [hmin, dtmax] with hmin = 16 ulp(t).  One call makes one trial: it returns
+// false when the trial was rejected (state unchanged, to be retried), true
+// when the step is done, either accepted or failed at hmin (state unchanged,
+// dt = hmin; the drivers do not distinguish the two, as the reference's do
+// not look at its -1 flag).
+CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                            const SolverParamsR &sp, realtype &dt, const realtype t1,
+                            realtype aux[], realtype wi[], RngData &rd, StepCtl &c)
 {
     const realtype EXPON = RCONST(1.0) / (CLODE_ERR_ORDER + RCONST(1.0));
     realtype tNew, normErr, err[N_VAR], newxi[N_VAR], newk1[N_VAR];
-    realtype newDt = dt;
     realtype threshold = sp.abstol / sp.reltol;
     realtype hmin = RCONST(16.0) * fabs(fabs(nextafter(ti, RCONST(1.1) * t1)) - ti);
-    bool noFailedSteps = true;
-    int attempts = 0;
-    while (true) {
-        if (++attempts > CLODE_MAX_ATTEMPTS) {
-            dt = hmin;
-            return -1;
-        }
-        tNew = ti;
-        for (int j = 0; j < N_VAR; j++) {
-            newxi[j] = xi[j];
-            newk1[j] = k1[j];
-        }
-        newDt = fmin(fmax(newDt, hmin), sp.dtmax);
-        newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, wi);
-        for (int j = 0; j < N_VAR; j++)
-            err[j] /= fmax(fmax(fabs(xi[j]), fabs(newxi[j])), threshold);
-        normErr = ZERO;
-        for (int j = 0; j < N_VAR; j++)
-            normErr = fmax(fabs(err[j]), normErr);
-        if (normErr > sp.reltol) {
-            if (newDt <= hmin) {
-                dt = hmin;
-                return -1;
-            }
-            if (noFailedSteps) {
-                noFailedSteps = false;
-                newDt *= fmax(CLODE_MAX_SHRINK, RCONST(0.8) * pow(sp.reltol / normErr, EXPON));
-            } else {
-                newDt *= RCONST(0.5);
-            }
-        } else {
-            break;
-        }
+    if (!c.inStep) {
+        c.inStep = true;
+        c.noFailedSteps = true;
+        c.attempts = 0;
+        c.newDt = dt;
     }
-    if (noFailedSteps)
-        newDt *= fmin(CLODE_MAX_GROW, RCONST(0.8) * pow(sp.reltol / normErr, EXPON));
+    if (++c.attempts > CLODE_MAX_ATTEMPTS) {
+        dt = hmin;
+        c.inStep = false;
+        return true;
+    }
+    tNew = ti;
+    for (int j = 0; j < N_VAR; j++) {
+        newxi[j] = xi[j];
+        newk1[j] = k1[j];
+    }
+    realtype newDt = fmin(fmax(c.newDt, hmin), sp.dtmax);
+    newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, wi);
+    for (int j = 0; j < N_VAR; j++)
+        err[j] /= fmax(fmax(fabs(xi[j]), fabs(newxi[j])), threshold);
+    normErr = ZERO;
+    for (int j = 0; j < N_VAR; j++)
+        normErr = fmax(fabs(err[j]), normErr);
+    // the factor both outcomes use, formed once for the whole wave
+    const realtype factor = RCONST(0.8) * pow(sp.reltol / normErr, EXPON);
+    if (normErr > sp.reltol) {
+        if (newDt <= hmin) {
+            dt = hmin;
+            c.inStep = false;
+            return true;
+        }
+        newDt *= c.noFailedSteps ? fmax(CLODE_MAX_SHRINK, factor) : RCONST(0.5);
+        c.noFailedSteps = false;
+        c.newDt = newDt;
+        return false;
+    }
+    if (c.noFailedSteps)
+        newDt *= fmin(CLODE_MAX_GROW, factor);
     newDt = fmin(newDt, t1 - tNew);
     newDt = fmin(fmax(newDt, hmin), sp.dtmax);
     dt = newDt;
@@ -294,7 +324,8 @@ CLODE_DEV int stepper(realtype &ti, realtype xi[], realtype k1[], const realtype
         xi[j] = newxi[j];
         k1[j] = newk1[j];
     }
-    return 0;
+    c.inStep = false;
+    return true;
 }
 #endif
 
@@ -366,9 +397,10 @@ CLODE_DEV void do_step(double &ti, realtype xi[], realtype xc[], realtype k1[], 
 #endif
 
 #if CLODE_FIXED_STEP
-CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
-                      const SolverParamsR &sp, double &dt, const double t1,
-                      realtype aux[], realtype wi[], RngData &rd, const double, const double, const double)
+CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                            const SolverParamsR &sp, double &dt, const double t1,
+                            realtype aux[], realtype wi[], RngData &rd, const double, const double, const double,
+                            StepCtl &)
 {
     do_step(ti, xi, xc, k1, pars, dt, aux, wi);
 #if CLODE_STOCHASTIC
@@ -377,7 +409,7 @@ CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], c
         wi[j] = rng_normal(rd) * rsq;
 #endif
     getRHS((realtype)ti, xi, pars, k1, aux, wi);
-    return 0;
+    return true;
 }
 #endif
 
@@ -471,57 +503,56 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
 #endif
 
 #if CLODE_ADAPTIVE_STEP
-// Same controller as the plain modes; the error ratio and its norm are
-// formed in double, the growth factor (rtol/err)^(1/(p+1)) with the
-// hardware log2/exp2 (it only steers the next step size).
-CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
-                      const SolverParamsR &sp, double &dt, const double t1,
-                      realtype aux[], realtype wi[], RngData &rd, const double reltol, const double threshold,
-                      const double dtmax)
+// Same controller as the plain modes, one trial per call; the error ratio
+// and its norm are formed in double, the growth factor (rtol/err)^(1/(p+1))
+// with the hardware log2/exp2 (it only steers the next step size).
+CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
+                            const SolverParamsR &sp, double &dt, const double t1,
+                            realtype aux[], realtype wi[], RngData &rd, const double reltol, const double threshold,
+                            const double dtmax, StepCtl &c)
 {
     double tNew, normErr, errd[N_VAR];
     realtype newxi[N_VAR], newk1[N_VAR], dx[N_VAR];
-    double newDt = dt;
     double hmin = 16.0 * fabs(fabs(nextafter(ti, 1.1 * t1)) - ti);
-    bool noFailedSteps = true;
-    int attempts = 0;
-    while (true) {
-        if (++attempts > CLODE_MAX_ATTEMPTS) {
+    if (!c.inStep) {
+        c.inStep = true;
+        c.noFailedSteps = true;
+        c.attempts = 0;
+        c.newDt = dt;
+    }
+    if (++c.attempts > CLODE_MAX_ATTEMPTS) {
+        dt = hmin;
+        c.inStep = false;
+        return true;
+    }
+    tNew = ti;
+    for (int j = 0; j < N_VAR; j++) {
+        newxi[j] = xi[j];
+        newk1[j] = k1[j];
+    }
+    double newDt = fmin(fmax(c.newDt, hmin), dtmax);
+    newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, errd, dx, wi);
+    normErr = 0.0;
+    for (int j = 0; j < N_VAR; j++) {
+        realtype sc = fmaxf(fmaxf(fabsf(xi[j]), fabsf(newxi[j])), (realtype)threshold);
+        normErr = fmax(fabs(errd[j]) * (double)__frcp_rn(sc), normErr);
+    }
+    // the factor both outcomes use (a rejected trial has normErr > reltol > 0)
+    const realtype q = normErr > 0.0 ? (realtype)(reltol / normErr) : 3.0e38f;
+    const double factor = (double)(0.8f * exp2f(CLODE_EXPON * log2f(q)));
+    if (normErr > reltol) {
+        if (newDt <= hmin) {
             dt = hmin;
-            return -1;
+            c.inStep = false;
+            return true;
         }
-        tNew = ti;
-        for (int j = 0; j < N_VAR; j++) {
-            newxi[j] = xi[j];
-            newk1[j] = k1[j];
-        }
-        newDt = fmin(fmax(newDt, hmin), dtmax);
-        newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, errd, dx, wi);
-        normErr = 0.0;
-        for (int j = 0; j < N_VAR; j++) {
-            realtype sc = fmaxf(fmaxf(fabsf(xi[j]), fabsf(newxi[j])), (realtype)threshold);
-            normErr = fmax(fabs(errd[j]) * (double)__frcp_rn(sc), normErr);
-        }
-        if (normErr > reltol) {
-            if (newDt <= hmin) {
-                dt = hmin;
-                return -1;
-            }
-            if (noFailedSteps) {
-                noFailedSteps = false;
-                realtype q = (realtype)(reltol / normErr);
-                newDt *= fmax(CLODE_MAX_SHRINK, (double)(0.8f * exp2f(CLODE_EXPON * log2f(q))));
-            } else {
-                newDt *= 0.5;
-            }
-        } else {
-            break;
-        }
+        newDt *= c.noFailedSteps ? fmax(CLODE_MAX_SHRINK, factor) : 0.5;
+        c.noFailedSteps = false;
+        c.newDt = newDt;
+        return false;
     }
-    if (noFailedSteps) {
-        realtype q = normErr > 0.0 ? (realtype)(reltol / normErr) : 3.0e38f;
-        newDt *= fmin(CLODE_MAX_GROW, (double)(0.8f * exp2f(CLODE_EXPON * log2f(q))));
-    }
+    if (c.noFailedSteps)
+        newDt *= fmin(CLODE_MAX_GROW, factor);
     newDt = fmin(newDt, t1 - tNew);
     newDt = fmin(fmax(newDt, hmin), dtmax);
     dt = newDt;
@@ -531,7 +562,8 @@ CLODE_DEV int stepper(double &ti, realtype xi[], realtype xc[], realtype k1[], c
         kahan_add(xi[j], xc[j], dx[j]);
         k1[j] = newk1[j];
     }
-    return 0;
+    c.inStep = false;
+    return true;
 }
 #endif
 
