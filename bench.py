@@ -363,7 +363,7 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
 def surrogate_c5(precision, n=2048):
     """2048 x 2048 lactotroph feature map (gK in [0, 10] x gBK in [0, 2],
     dopri5 rtol 1e-6, transient 5e3 then thresh2 over 2e4): the plain map
-    against the surrogate-screened map (2^18 integrated training samples,
+    against the surrogate-screened map (2^17 integrated training samples,
     BF16-MFMA MLP, screening, re-solve of the flagged cells)."""
     import clode_amd as clode
     from clode_amd.surrogate import compare_maps, integrator_solver, screened_map

@@ -78,8 +78,12 @@ __device__ inline float dsilu(float z)
 // C[M x N] = A[M x K] * B[K x N] with arbitrary strides on both operands,
 // BF16 operands, FP32 accumulation on v_mfma_f32_16x16x32_bf16, and a fused
 // epilogue.  Workgroup tile 64 x 64 (4 waves of 32 x 32, each 2 x 2 MFMA
-// blocks), K step 32.  Both tiles are staged in LDS with k contiguous, so each
-// lane's 8-element operand is one 16-byte LDS read.
+// blocks), MFMA K step 32.  K is staged through LDS in chunks of 128 with k
+// contiguous (each lane's 8-element operand is one 16-byte LDS read; the row
+// pitch of 136 elements puts the 16 rows a read instruction touches on
+// disjoint banks), and the global loads of the next chunk are issued before
+// the MFMAs of the current one, so a 256-deep product waits on memory about
+// once rather than once per K step.
 enum Epi { EPI_FWD_SILU = 0, EPI_FWD_LINEAR = 1, EPI_BWD_DSILU = 2, EPI_WGRAD = 3 };
 
 struct GemmArgs {
@@ -99,13 +103,46 @@ struct GemmArgs {
     long long cstride;  // Cout offset between splits (EPI_WGRAD)
 };
 
-constexpr int TM = 64, TN = 64, TK = 32, KPAD = 8;
+constexpr int TM = 64, TN = 64, TK = 32, KC = 128, KPAD = 8;
+constexpr int KCP = KC + KPAD;
+constexpr int KG = KC / 8;                 // 16-byte groups per staged row
+constexpr int LD_PER_T = TM * KG / 256;    // 16-byte loads per thread per operand and chunk
+
+// rows [r0, r0 + 64) x k [k, k + KC) of a strided BF16 operand into registers
+// (zero outside [0, rows) x [k, kend)); slot j of thread t is row
+// (t + 256 j) / KG, group (t + 256 j) % KG, so 16 consecutive threads read one
+// contiguous 256-byte row segment
+__device__ inline void gemm_load(const bf16_t *X, long long ld_r, long long ld_k, bool vec, int rows, int r0,
+                                 int k, int kend, u16x8 (&v)[LD_PER_T])
+{
+#pragma unroll
+    for (int j = 0; j < LD_PER_T; ++j) {
+        const int slot = (int)threadIdx.x + 256 * j;
+        const int r = r0 + slot / KG, kk = k + (slot % KG) * 8;
+        if (vec && r < rows && kk + 8 <= kend) {
+            v[j] = *(const u16x8 *)(X + (long long)r * ld_r + kk);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 8; ++e)
+                v[j][e] = (r < rows && kk + e < kend) ? X[(long long)r * ld_r + (long long)(kk + e) * ld_k] : 0;
+        }
+    }
+}
+
+__device__ inline void gemm_stage(bf16_t (*S)[KCP], const u16x8 (&v)[LD_PER_T])
+{
+#pragma unroll
+    for (int j = 0; j < LD_PER_T; ++j) {
+        const int slot = (int)threadIdx.x + 256 * j;
+        *(u16x8 *)&S[slot / KG][(slot % KG) * 8] = v[j];
+    }
+}
 
 template <int EPI>
 __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t As[TM][TK + KPAD];
-    __shared__ __attribute__((aligned(16))) bf16_t Bs[TN][TK + KPAD];
+    __shared__ __attribute__((aligned(16))) bf16_t As[TM][KCP];
+    __shared__ __attribute__((aligned(16))) bf16_t Bs[TN][KCP];
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
@@ -119,47 +156,36 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
         for (int j = 0; j < 2; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    // loader mapping: thread t stages row/col (t >> 2), k (t & 3) * 8 .. + 7
-    const int lr = t >> 2, lk = (t & 3) * 8;
     const bool a_vec = g.lda_k == 1 && (g.lda_m % 8) == 0 && ((uintptr_t)g.A & 15) == 0;
     const bool b_vec = g.ldb_k == 1 && (g.ldb_n % 8) == 0 && ((uintptr_t)g.B & 15) == 0;
-    for (int k0 = kbeg; k0 < kend; k0 += TK) {
-        {
-            const int m = m0 + lr, k = k0 + lk;
-            u16x8 v;
-            if (a_vec && m < g.M && k + 8 <= kend) {
-                v = *(const u16x8 *)(g.A + (long long)m * g.lda_m + k);
-            } else {
-#pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    v[s] = (m < g.M && k + s < kend) ? g.A[(long long)m * g.lda_m + (long long)(k + s) * g.lda_k] : 0;
-            }
-            *(u16x8 *)&As[lr][lk] = v;
-        }
-        {
-            const int n = n0 + lr, k = k0 + lk;
-            u16x8 v;
-            if (b_vec && n < g.N && k + 8 <= kend) {
-                v = *(const u16x8 *)(g.B + (long long)n * g.ldb_n + k);
-            } else {
-#pragma unroll
-                for (int s = 0; s < 8; ++s)
-                    v[s] = (n < g.N && k + s < kend) ? g.B[(long long)(k + s) * g.ldb_k + (long long)n * g.ldb_n] : 0;
-            }
-            *(u16x8 *)&Bs[lr][lk] = v;
-        }
+    u16x8 va[LD_PER_T], vb[LD_PER_T];
+    if (kbeg < kend) {
+        gemm_load(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kbeg, kend, va);
+        gemm_load(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kbeg, kend, vb);
+    }
+    for (int kc = kbeg; kc < kend; kc += KC) {
+        gemm_stage(As, va);
+        gemm_stage(Bs, vb);
         __syncthreads();
-        const int r = lane & 15, kk = (lane >> 4) * 8;
+        if (kc + KC < kend) {  // next chunk in flight during this chunk's MFMAs
+            gemm_load(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kc + KC, kend, va);
+            gemm_load(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kc + KC, kend, vb);
+        }
+        const int nks = (min(KC, kend - kc) + TK - 1) / TK;
+        const int r = lane & 15;
+        for (int ks = 0; ks < nks; ++ks) {
+            const int kk = ks * TK + (lane >> 4) * 8;
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-            bf16x8 a = *(const bf16x8 *)&As[wm * 32 + i * 16 + r][kk];
+            for (int i = 0; i < 2; ++i) {
+                bf16x8 a = *(const bf16x8 *)&As[wm * 32 + i * 16 + r][kk];
 #pragma unroll
-            for (int j = 0; j < 2; ++j) {
-                bf16x8 b = *(const bf16x8 *)&Bs[wn * 32 + j * 16 + r][kk];
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+                for (int j = 0; j < 2; ++j) {
+                    bf16x8 b = *(const bf16x8 *)&Bs[wn * 32 + j * 16 + r][kk];
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
+                }
             }
         }
-        __syncthreads();
+        __syncthreads();  // before the next chunk overwrites the tiles
     }
     // epilogue: lane holds rows (lane >> 4) * 4 + q, column lane & 15 of each block
 #pragma unroll
@@ -194,14 +220,9 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
 }
 
 // ---------------------------------------------------------------- encode
-// A0t (optional): transposed copy [32][ldt]
-__global__ void __launch_bounds__(256) mlp_encode(const float *X, int n, const float *Bff, bf16_t *A0, bf16_t *A0t,
-                                                  int ldt)
+// row i of the encoded inputs; A0t (optional): transposed copy [32][ldt]
+__device__ inline void encode_row(float x0, float x1, int i, const float *Bff, bf16_t *A0, bf16_t *A0t, int ldt)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const float x0 = X[2 * i], x1 = X[2 * i + 1];
     for (int f = 0; f < NFF; ++f) {
         const float p = 6.283185307179586f * (Bff[f] * x0 + Bff[NFF + f] * x1);
         const bf16_t s = f2bf(sinf(p)), c = f2bf(cosf(p));
@@ -212,6 +233,15 @@ __global__ void __launch_bounds__(256) mlp_encode(const float *X, int n, const f
             A0t[(long long)(2 * f + 1) * ldt + i] = c;
         }
     }
+}
+
+__global__ void __launch_bounds__(256) mlp_encode(const float *X, int n, const float *Bff, bf16_t *A0, bf16_t *A0t,
+                                                  int ldt)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    encode_row(X[2 * i], X[2 * i + 1], i, Bff, A0, A0t, ldt);
 }
 
 // grid inputs: cell c = ix * ny + iy at (lo + (hi - lo) * i / (n - 1)), normalised to [0, 1]
@@ -239,30 +269,34 @@ __host__ __device__ inline bool head_keeps(unsigned long long sample, int head)
 // Per sample and head: squared error of the two normalised regression
 // targets plus the softmax cross-entropy of the class, over the samples the
 // head keeps; the loss is the mean over the batch of the sum over heads.
-// dOut (BF16, [n][128]) is the gradient of that loss w.r.t. the head outputs.
+// dOut (BF16, [n][128]) is the gradient of that loss w.r.t. the head outputs,
+// dOutT ([128][ldt]) its transpose.  One thread per (sample, head), sample
+// fastest, so the transposed rows are written coalesced; the columns past the
+// heads stay zero in dOut (cleared when the buffer is allocated) and are
+// cleared here in dOutT, whose row pitch follows the batch size.
+constexpr int PADROWS = (NOUTP - NOUT) / NHEAD;  // transposed padding rows each head thread clears
+static_assert(PADROWS * NHEAD == NOUTP - NOUT, "padding rows split evenly over the heads");
+
 __global__ void __launch_bounds__(256) mlp_loss(const float *Zh, int n, const float *T, const int *cls,
                                                 const unsigned long long *ids, float inv_n, bf16_t *dOut,
                                                 bf16_t *dOutT, int ldt, float *loss)
 {
     __shared__ float red[256];
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     float l = 0.f;
-    if (i < n) {
-        const float *z = Zh + (long long)i * NOUTP;
-        bf16_t *d = dOut + (long long)i * NOUTP;
+    if (t < (long long)NHEAD * n) {
+        const int h = (int)(t / n), i = (int)(t % n);
+        const float *zh = Zh + (long long)i * NOUTP + h * HW;
+        bf16_t d[HW];
         const unsigned long long id = ids ? ids[i] : (unsigned long long)i;
-        for (int h = 0; h < NHEAD; ++h) {
-            const float *zh = z + h * HW;
-            bf16_t *dh = d + h * HW;
-            if (!head_keeps(id, h)) {
-                for (int c = 0; c < HW; ++c)
-                    dh[c] = 0;
-                continue;
-            }
+        if (!head_keeps(id, h)) {
+            for (int c = 0; c < HW; ++c)
+                d[c] = 0;
+        } else {
             const float e0 = zh[0] - T[2 * i], e1 = zh[1] - T[2 * i + 1];
             l += e0 * e0 + e1 * e1;
-            dh[0] = f2bf(2.f * e0 * inv_n);
-            dh[1] = f2bf(2.f * e1 * inv_n);
+            d[0] = f2bf(2.f * e0 * inv_n);
+            d[1] = f2bf(2.f * e1 * inv_n);
             float mx = zh[2];
             for (int c = 1; c < NCLS; ++c)
                 mx = fmaxf(mx, zh[2 + c]);
@@ -273,13 +307,16 @@ __global__ void __launch_bounds__(256) mlp_loss(const float *Zh, int n, const fl
             l += (mx + __logf(se)) - zh[2 + y];
             for (int c = 0; c < NCLS; ++c) {
                 const float p = __expf(zh[2 + c] - mx) / se;
-                dh[2 + c] = f2bf((p - (c == y ? 1.f : 0.f)) * inv_n);
+                d[2 + c] = f2bf((p - (c == y ? 1.f : 0.f)) * inv_n);
             }
         }
-        for (int c = NOUT; c < NOUTP; ++c)
-            d[c] = 0;
-        for (int c = 0; c < NOUTP; ++c)
-            dOutT[(long long)c * ldt + i] = d[c];
+        bf16_t *dh = dOut + (long long)i * NOUTP + h * HW;
+        for (int c = 0; c < HW; ++c) {
+            dh[c] = d[c];
+            dOutT[(long long)(h * HW + c) * ldt + i] = d[c];
+        }
+        for (int c = NOUT + h * PADROWS; c < NOUT + (h + 1) * PADROWS; ++c)
+            dOutT[(long long)c * ldt + i] = 0;
     }
     red[threadIdx.x] = l;
     __syncthreads();
@@ -292,12 +329,27 @@ __global__ void __launch_bounds__(256) mlp_loss(const float *Zh, int n, const fl
         atomicAdd(loss, red[0] * inv_n);
 }
 
-// bias gradient: db[c] = sum_m dZ[m][c], read from the transposed copy
-// dZt[c][m] (BF16, row stride ld) so that each block reads one contiguous row
-__global__ void __launch_bounds__(256) mlp_colsum(const bf16_t *dZt, int n, int ncol, int ld, float *db)
+// one launch for both per-layer reductions of the backward pass: blocks
+// [0, sum_blocks) sum the split-K partial weight gradients in a fixed order
+// (gw[e] = sum_z part[z * count + e]); the remaining ncol blocks form the bias
+// gradient db[c] = sum_m dZ[m][c] from the transposed copy dZt[c][m] (row
+// stride ld), each block reading one contiguous row
+__global__ void __launch_bounds__(256) mlp_layer_grads(const float *part, int splits, long long count, float *gw,
+                                                       int sum_blocks, const bf16_t *dZt, int n, int ncol, int ld,
+                                                       float *db)
 {
     __shared__ float red[256];
-    const int c = blockIdx.x;
+    if ((int)blockIdx.x < sum_blocks) {
+        const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (e >= count)
+            return;
+        float s = 0.f;
+        for (int z = 0; z < splits; ++z)
+            s += part[(long long)z * count + e];
+        gw[e] = s;
+        return;
+    }
+    const int c = (int)blockIdx.x - sum_blocks;
     float s = 0.f;
     for (int m = threadIdx.x; m < n; m += blockDim.x)
         s += bf2f(dZt[(long long)c * ld + m]);
@@ -310,18 +362,6 @@ __global__ void __launch_bounds__(256) mlp_colsum(const bf16_t *dZt, int n, int 
     }
     if (threadIdx.x == 0 && c < ncol)
         db[c] = red[0];
-}
-
-// sum of the split-K partial products: out[e] = sum_z part[z * count + e]
-__global__ void __launch_bounds__(256) mlp_split_sum(const float *part, int splits, long long count, float *out)
-{
-    const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= count)
-        return;
-    float s = 0.f;
-    for (int z = 0; z < splits; ++z)
-        s += part[(long long)z * count + e];
-    out[e] = s;
 }
 
 // layer table for the Adam kernel: weights of layer l occupy
@@ -359,20 +399,21 @@ __global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float 
     }
 }
 
-// minibatch gather: row r of the batch is sample perm[r] of the training set
+// minibatch gather: row r of the batch is sample perm[r] of the training
+// set; its inputs are encoded (with the transposed copy) in the same pass
 __global__ void __launch_bounds__(256) mlp_gather(const unsigned int *perm, int nb, const float *Xa, const float *Ta,
-                                                  const int *Ca, float *X, float *T, int *C, unsigned long long *ids)
+                                                  const int *Ca, float *T, int *C, unsigned long long *ids,
+                                                  const float *Bff, bf16_t *A0, bf16_t *A0t, int ldt)
 {
     const int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= nb)
         return;
     const unsigned int j = perm[r];
-    X[2 * r] = Xa[2 * j];
-    X[2 * r + 1] = Xa[2 * j + 1];
     T[2 * r] = Ta[2 * j];
     T[2 * r + 1] = Ta[2 * j + 1];
     C[r] = Ca[j];
     ids[r] = j;
+    encode_row(Xa[2 * j], Xa[2 * j + 1], r, Bff, A0, A0t, ldt);
 }
 
 // ---------------------------------------------------------------- screening
@@ -542,6 +583,8 @@ struct clode_mlp {
             Z[l].alloc(n * H);
         }
         dOut.alloc(n * NOUTP);
+        // the loss kernel never writes the padding columns: they stay zero
+        check(hipMemsetAsync(dOut.p, 0, n * NOUTP * sizeof(bf16_t), stream), "hipMemsetAsync");
         dOutT.alloc(nt * NOUTP);
         Zh.alloc(n * NOUTP);
         X.alloc(n * 2);
@@ -628,11 +671,10 @@ void backward(clode_mlp *h, int n)
         const long long count = (long long)Ly.out * Ly.in;
         w.Cout = h->wsp.p; w.ldc = Ly.in; w.cstride = count;
         gemm(h, EPI_WGRAD, w);
-        hipLaunchKernelGGL(mlp_split_sum, dim3((unsigned int)((count + 255) / 256)), dim3(256), 0, h->stream, h->wsp.p,
-                           used, count, h->g.p + Ly.woff);
-        check(hipGetLastError(), "mlp_split_sum launch");
-        hipLaunchKernelGGL(mlp_colsum, dim3(Ly.out), dim3(256), 0, h->stream, dzt, n, Ly.out, lt, h->g.p + Ly.boff);
-        check(hipGetLastError(), "mlp_colsum launch");
+        const int sum_blocks = (int)((count + 255) / 256);
+        hipLaunchKernelGGL(mlp_layer_grads, dim3((unsigned int)(sum_blocks + Ly.out)), dim3(256), 0, h->stream,
+                           h->wsp.p, used, count, h->g.p + Ly.woff, sum_blocks, dzt, n, Ly.out, lt, h->g.p + Ly.boff);
+        check(hipGetLastError(), "mlp_layer_grads launch");
         if (l == 0)
             break;
         // dY[l-1][m][i] = (sum_o dz[m][o] W[i][o]) * silu'(Z[l-1][m][i])
@@ -691,7 +733,7 @@ float loss_grad(clode_mlp *h, int n)
     encode(h, h->X.p, n, true);
     forward(h, n, true);
     check(hipMemsetAsync(h->loss.p, 0, 4, h->stream), "hipMemset");
-    hipLaunchKernelGGL(mlp_loss, dim3((n + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, n, h->T.p, h->C.p, h->ids.p,
+    hipLaunchKernelGGL(mlp_loss, dim3((NHEAD * n + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, n, h->T.p, h->C.p, h->ids.p,
                        1.0f / (float)n, h->dOut.p, h->dOutT.p, ldT(n), h->loss.p);
     check(hipGetLastError(), "mlp_loss launch");
     backward(h, n);
@@ -961,11 +1003,11 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
             const size_t b0 = s * (size_t)batch;
             const int nb = (int)std::min<size_t>((size_t)batch, n - b0);
             hipLaunchKernelGGL(mlp_gather, dim3((nb + 255) / 256), dim3(256), 0, h->stream, dperm.p + b0, nb,
-                               Xall.p, Tall.p, Call.p, h->X.p, h->T.p, h->C.p, h->ids.p);
+                               Xall.p, Tall.p, Call.p, h->T.p, h->C.p, h->ids.p, h->dBff.p, h->A0.p, h->A0t.p,
+                               ldT(nb));
             check(hipGetLastError(), "mlp_gather launch");
-            encode(h, h->X.p, nb, true);
             forward(h, nb, true);
-            hipLaunchKernelGGL(mlp_loss, dim3((nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->T.p,
+            hipLaunchKernelGGL(mlp_loss, dim3((NHEAD * nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->T.p,
                                h->C.p, h->ids.p, 1.0f / (float)nb, h->dOut.p, h->dOutT.p, ldT(nb), eloss.p);
             check(hipGetLastError(), "mlp_loss launch");
             backward(h, nb);
