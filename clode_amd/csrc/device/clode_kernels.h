@@ -361,7 +361,8 @@ struct TrajectoryLane {
         member_tspan(a, i, t0, t1);
         begin_member(m, a, i, t0);
         storeix = 0;
-        trajectory_store(a, i, storeix, m);
+        if (a.sp_max_store > 0)
+            trajectory_store(a, i, storeix, m);
         step = 0;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const

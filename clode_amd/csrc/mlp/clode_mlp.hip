@@ -94,7 +94,7 @@ struct GemmArgs {
     long long ldb_k, ldb_n;
     const float *bias;  // [N]                      (forward)
     const float *Zin;   // [M][ldz] pre-activation  (EPI_BWD_DSILU)
-    float *Zout;        // [M][ldz] pre-activation  (forward)
+    float *Zout;        // [M][ldz] pre-activation  (forward; optional with SiLU)
     bf16_t *Yout;       // [M][ldy]                 (forward SiLU, dsilu)
     float *Cout;        // [M][ldc]                 (EPI_WGRAD)
     bf16_t *YTout;      // [N][ldt] transposed copy of Yout (optional)
@@ -201,7 +201,8 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
                 float v = acc[i][j][q];
                 if (EPI == EPI_FWD_SILU || EPI == EPI_FWD_LINEAR) {
                     v += g.bias[n];
-                    g.Zout[(long long)m * g.ldz + n] = v;
+                    if (EPI == EPI_FWD_LINEAR || g.Zout)
+                        g.Zout[(long long)m * g.ldz + n] = v;
                     if (EPI == EPI_FWD_SILU) {
                         const bf16_t y = f2bf(silu(v));
                         g.Yout[(long long)m * g.ldy + n] = y;
@@ -629,9 +630,10 @@ void forward(clode_mlp *h, int n, bool train = false)
         a.A = in; a.lda_m = kin; a.lda_k = 1;
         a.B = h->pbf.p + h->L[l].woff; a.ldb_k = 1; a.ldb_n = kin;
         a.bias = h->p.p + h->L[l].boff;
-        a.Zout = h->Z[l].p; a.ldz = H;
+        a.ldz = H;
         a.Yout = h->Y[l].p; a.ldy = H;
-        if (train) {
+        if (train) {  // backward needs the pre-activations and the transposed activations
+            a.Zout = h->Z[l].p;
             a.YTout = h->Yt[l].p;
             a.ldt = ldT(n);
         }
