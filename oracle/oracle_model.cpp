@@ -335,7 +335,8 @@ struct Ensemble {
         };
         const R t1 = mt1(i);
         unsigned int storeix = 0;
-        put(storeix);
+        if (sp.max_store > 0)  // max_store 0: no sample buffer at all
+            put(storeix);
         unsigned int step = 0;
         while (m.ti <= t1 && step < sp.max_steps && storeix < sp.max_store) {
             ++step;

@@ -140,6 +140,25 @@ def test_trajectory_max_store_truncation(gpu_available):
     assert np.allclose(tf, 0.5, atol=1e-9)  # 50 steps of 0.01
 
 
+@pytest.mark.parametrize("max_store", [0, 1, 2])
+def test_trajectory_tiny_max_store(gpu_available, max_store):
+    """max_store 0 writes no sample and takes no step (the loop runs while
+    storeix < max_store); 1 and 2 stop after one and two steps."""
+    sp = (0.01, 1.0, 1e-6, 1e-3, 10000000, max_store, 1)
+    p = Pair("van_der_pol", "rk4", "", "double", kind="trajectory", sp=sp)
+    x0, pars = vdp_data(70)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 10.0)
+    p.trajectory()
+    p.assert_bitexact(("xf", "tf"))
+    ns = np.array(p.gpu.get_n_stored())
+    assert np.all(ns == max_store)
+    if max_store == 0:
+        assert np.array_equal(np.array(p.gpu.get_xf()), x0.ravel())
+    else:
+        p.assert_trajectory_bitexact()
+
+
 def test_seeded_noise_stream(gpu_available):
     p = Pair("ornstein_uhlenbeck", "seuler", "", "double", kind="base",
              sp=(0.01, 0.01, 1e-6, 1e-3, 1000000, 1000000, 1))

@@ -162,6 +162,12 @@ def test_unknown_names_and_bad_input():
     s = w.SimulatorBase(pi, "rk4", False, w.OpenCLResource(), "")
     s.set_stepper("nosuch")  # reference: warning, stepper unchanged
     assert "#define CLODE_STEPPER 3" in s.get_program_string()
+    with pytest.raises(ValueError, match="at least one member"):
+        s.set_problem_data([], [])  # empty ensemble, refused before any device work
+    bad = w.SolverParams()
+    bad.nout = 0
+    with pytest.raises(ValueError, match="nout"):
+        s.set_solver_params(bad)
     with pytest.raises(RuntimeError):
         s.set_problem_data([1.0, 2.0], [1.0])  # no device here
 
