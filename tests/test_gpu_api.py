@@ -132,3 +132,48 @@ def test_ensemble_construction_semantics(gpu_available):
     sim.shift_tspan()
     assert sim.get_tspan() == (0.5, 1.0)
     assert "rk4" in sim.get_available_steppers()
+
+
+def harmonic_rhs(t: float, vars: List[float], p: List[float], dy: List[float], aux: List[float],
+                 w: List[float]) -> None:
+    k: float = p[0]
+    x: float = vars[0]
+    y: float = vars[1]
+    dy[0] = y
+    dy[1] = -k * x
+
+
+@pytest.mark.parametrize("observer", list(clode.Observer))
+def test_observers_on_the_unit_circle(gpu_available, observer):
+    """The harmonic-oscillator cases of test/test_observers.py:47-138 (a
+    script there; assertions here): x = -cos t, y = sin t from (-1, 0), rk4
+    dt 0.01 over 10 periods, feature variable y."""
+    sim = clode.FeatureSimulator(rhs_equation=harmonic_rhs, variables={"x": -1.0, "y": 0.0}, parameters={"k": 1.0},
+                                 dt=0.01, t_span=(0.0, 10 * 2 * math.pi), observer=observer, event_var="y",
+                                 feature_var="y", observer_min_x_amp=0.0, observer_x_up_thresh=0.3,
+                                 observer_x_down_thresh=0.2, observer_neighbourhood_radius=0.05,
+                                 observer_max_event_timestamps=10, single_precision=False)
+    out = sim.features()
+    names = out.get_feature_names()
+    state = sim.get_final_state()[0]
+    # back at the start after 10 periods (the last step ends up to dt past 20 pi)
+    assert np.allclose(state, [-1.0, 0.0], atol=0.02)
+    if "max x" in names:
+        assert out.get_var_max("x") == pytest.approx(1.0, abs=1e-6)
+        assert out.get_var_min("x") == pytest.approx(-1.0, abs=1e-6)
+    if "mean x" in names:
+        assert out.get_var_mean("x") == pytest.approx(0.0, abs=2e-3)
+        assert out.get_var_mean("y") == pytest.approx(0.0, abs=2e-3)
+    if "mean amplitude" in names:
+        assert out.get_var_mean("amplitude") == pytest.approx(2.0, abs=1e-3)
+    if "mean IMI" in names:
+        for stat in (out.get_var_max, out.get_var_min, out.get_var_mean):
+            assert stat("IMI") == pytest.approx(2 * math.pi, abs=0.02)
+    if "mean period" in names:
+        assert out.get_var_mean("period") == pytest.approx(2 * math.pi, abs=0.02)
+    if "event count" in names:
+        # one event per period (the neighbourhood observers start counting at
+        # their trigger point, so they see one fewer)
+        assert int(out.get_var_count("event")) in (9, 10)
+    if "step count" in names:
+        assert int(out.get_var_count("step")) == int(10 * 2 * math.pi / 0.01) + 1
