@@ -208,13 +208,23 @@ struct DeviceSelector {
             return "";
         return device_info(device).version;
     }
+    // OpenCLResource::print (OpenCLResource.cpp:313-319): info-level log
+    // lines, so nothing is printed above LogLevel.info
     void print_devices()
     {
+        int level = 0;
+        clode_get_log_level(&level);
+        if (level > 2)  // info
+            return;
         int n = 0;
         clode_device_count(&n);
+        py::print("Selected platform and device: ");
+        py::print("Platform  --------------------");
+        py::print("Name: HIP (AMD ROCm), selected through the OpenCLResource interface");
         for (int d = 0; d < n; ++d) {
             DeviceInfoPy i = device_info(d);
-            py::print("device", d, ":", i.name, i.version, "CUs", i.computeUnits, "memory", i.deviceMemSize);
+            py::print(d == device ? "* device" : "  device", d, ":", i.name, i.version, "CUs", i.computeUnits,
+                      "memory", i.deviceMemSize);
         }
     }
 };

@@ -177,3 +177,39 @@ def test_observers_on_the_unit_circle(gpu_available, observer):
         assert int(out.get_var_count("event")) in (9, 10)
     if "step count" in names:
         assert int(out.get_var_count("step")) == int(10 * 2 * math.pi / 0.01) + 1
+
+
+def test_logger_and_device_printing(gpu_available, capfd):
+    """test/test_logger.py:4-28: device printing follows the log level."""
+    sim = clode.TrajectorySimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"), variables={"x": 0.0, "y": 1.0},
+                                    parameters={"mu": 1.0}, num_noise=0, stepper=clode.Stepper.dormand_prince,
+                                    device_id=0, platform_id=0)
+    old = clode.get_log_level()
+    try:
+        clode.set_log_level(clode.LogLevel.trace)
+        assert clode.get_log_level() == clode.LogLevel.trace
+        sim.print_devices()
+        captured = capfd.readouterr()
+        assert "OpenCL" in captured.out and "gfx950" in captured.out
+        assert captured.err == ""
+        clode.set_log_level(clode.LogLevel.off)
+        assert clode.get_log_level() == clode.LogLevel.off
+        sim.print_devices()
+        captured = capfd.readouterr()
+        assert captured.out == "" and captured.err == ""
+    finally:
+        clode.set_log_level(old)
+
+
+def test_vdp_dormand_prince_long(gpu_available):
+    """test/test_vdp_long.py:10-11: the Van der Pol period for mu up to 100."""
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "van_der_pol.cl"), variables={"x": 1.0, "y": 1.0},
+                                 parameters={"mu": 1.0}, observer=clode.Observer.threshold_2,
+                                 stepper=clode.Stepper.dormand_prince, t_span=(0.0, 1000.0), max_store=20000,
+                                 max_steps=20000)
+    mus = [-1, 0, 0.01, 0.1, 0.5, 1.0, 1.5, 2.0, 2.5, 3.0, 3.5, 4.0] + list(range(5, 101))
+    sim.set_ensemble(parameters={"mu": mus})
+    sim.features()
+    periods = sim.get_observer_results().get_var_max("period")
+    for i, mu in enumerate(mus):
+        assert np.isclose(periods[i], vdp_period(mu), rtol=0.01, atol=1), (mu, periods[i])
