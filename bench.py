@@ -17,7 +17,12 @@ HIP-event kernel times and the algorithmic FLOPs of clode_amd/perfmodel.py,
 and a CPU baseline: the CPU oracle (the reference-semantics restatement,
 not the reference binary, which cannot run here) timed on a member sample.
 
-Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra]
+Extra workloads (--extras, default all): fp32c (the headline in fp32c), c1
+(Van der Pol 4096, rk4), noise (the lactotroph Euler-Maruyama variant),
+c3 (Lorenz 2^20, and the 65,536 x 1024-sample trajectory), c4 (mixed 2^20
+batched sweep), c5 (surrogate-screened 2048 x 2048 map).
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extra] [--extras LIST]
 """
 from __future__ import annotations
 
@@ -208,6 +213,50 @@ def vdp_c1(n=4096, cpu_members=256):
     cs += o.last_steps + o.last_warm_steps
     cw = time.perf_counter() - t0
     res["cpu_oracle"] = {"members": int(idx.size), "threads": threads, "member_steps_per_s": cs / cw}
+    return res
+
+
+def noise_c2(n=65536, seed=2024):
+    """The lactotroph noise variant: 65,536 identical members (gcal 1.5, gsk 3,
+    gbk 1, sigma 2 in lactotroph_noise.cl), Euler-Maruyama dt 0.05, seeded;
+    transient over 5e3 ms, then thresh2 warm-up and features over 2e4 ms, in
+    FP64 and fp32c.  Member-steps/s of the sequence and the spread of the
+    mean period across the noise realisations."""
+    import clode_amd as clode
+
+    res = {}
+    for prec in ("double", "fp32c"):
+        sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lactotroph_noise.cl"),
+                                     variables={"v": -60.0, "n": 0.0, "f": 0.0, "c": 0.1},
+                                     parameters={"gcal": 1.5, "gsk": 3.0, "gbk": 1.0, "sigma": 2.0}, aux=["ica"],
+                                     num_noise=1, observer=clode.Observer.threshold_2,
+                                     stepper=clode.Stepper.stochastic_euler, single_precision=prec, dt=0.05,
+                                     dtmax=0.05, max_steps=10 ** 8, observer_x_up_thresh=0.3,
+                                     observer_x_down_thresh=0.2)
+        sim.set_repeat_ensemble(n)
+        sim.seed_rng(seed)
+        sim.enable_step_counter(True)
+        sim.transient(t_span=(0.0, 100.0))  # compile and first launch, untimed
+        sim.set_repeat_ensemble(n)
+        sim.seed_rng(seed)
+        sim.reset_step_counts()
+        t0 = time.perf_counter()
+        sim.transient(t_span=(0.0, 5000.0))
+        k_tr = sim.last_kernel_ms()
+        sim.set_tspan((0.0, 20000.0))
+        sim.initialize_observer()
+        k_init = sim.last_kernel_ms()
+        sim.features(initialize_observer=False, fetch_results=False)
+        k_feat = sim.last_kernel_ms()
+        wall = time.perf_counter() - t0
+        steps = int(sim.get_step_counts().sum())
+        period = np.ravel(sim.get_observer_results().get_var_mean("period"))
+        ok = period > 0
+        res[prec] = {"member_steps": steps, "wall_s": wall, "member_steps_per_s": steps / wall,
+                     "kernel_ms": {"transient": k_tr, "warmup": k_init, "features": k_feat},
+                     "mean_period_ms": float(period[ok].mean()) if ok.any() else 0.0,
+                     "period_cv": float(period[ok].std() / period[ok].mean()) if ok.any() else 0.0}
+    res["fp32c_speedup_wall"] = res["double"]["wall_s"] / res["fp32c"]["wall_s"]
     return res
 
 
@@ -414,8 +463,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-members", type=int, default=1024)
-    ap.add_argument("--extras", default="fp32c,c1,c3,c4,c5",
-                    help="comma list of extra workloads: fp32c, c1, c3, c4, c5 (or 'none')")
+    ap.add_argument("--extras", default="fp32c,c1,noise,c3,c4,c5",
+                    help="comma list of extra workloads: fp32c, c1, noise, c3, c4, c5 (or 'none'); --no-extra "
+                         "drops them all")
     ap.add_argument("--c4-rounds", type=int, default=20)
     ap.add_argument("--mlp-kernels", action="store_true",
                     help="run only the surrogate kernels (one training epoch over 65,536 synthetic samples and one "
@@ -492,6 +542,8 @@ def main():
         }
     if "c1" in wanted:
         run_extra("vdp_4096_rk4_fp64", vdp_c1)
+    if "noise" in wanted:
+        run_extra("lactotroph_noise_65536_seuler", noise_c2)
     if "c3" in wanted:
         run_extra("lorenz_2p20_localmax", lorenz_c3)
     if "c4" in wanted:
