@@ -187,15 +187,25 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
         }
         __syncthreads();  // before the next chunk overwrites the tiles
     }
-    // epilogue: lane holds rows (lane >> 4) * 4 + q, column lane & 15 of each block
+    // epilogue: lane holds rows (lane >> 4) * 4 + q, column lane & 15 of each
+    // block.  The transposed copy (YTout) is first gathered in LDS (the
+    // tiles are free once the K loop's last barrier has passed) and then
+    // written a row of YT at a time with 16-byte stores, instead of one
+    // scattered 2-byte store per element.
+    constexpr int TP = TM + 8;  // pitch of the transposed tile
+    static_assert(TN * TP <= TM * KCP, "the transposed tile fits in the A tile's LDS");
+    bf16_t(*Tt)[TP] = reinterpret_cast<bf16_t(*)[TP]>(&As[0][0]);
+    const bool stage_t = (EPI == EPI_FWD_SILU || EPI == EPI_BWD_DSILU) && g.YTout != nullptr;
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int m = m0 + wm * 32 + i * 16 + (lane >> 4) * 4 + q;
-                const int n = n0 + wn * 32 + j * 16 + (lane & 15);
+                const int ml = wm * 32 + i * 16 + (lane >> 4) * 4 + q;
+                const int nl = wn * 32 + j * 16 + (lane & 15);
+                const int m = m0 + ml;
+                const int n = n0 + nl;
                 if (m >= g.M || n >= g.N)
                     continue;
                 float v = acc[i][j][q];
@@ -206,18 +216,37 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
                     if (EPI == EPI_FWD_SILU) {
                         const bf16_t y = f2bf(silu(v));
                         g.Yout[(long long)m * g.ldy + n] = y;
-                        if (g.YTout)
-                            g.YTout[(long long)n * g.ldt + m] = y;
+                        if (stage_t)
+                            Tt[nl][ml] = y;
                     }
                 } else if (EPI == EPI_BWD_DSILU) {
                     const bf16_t y = f2bf(v * dsilu(g.Zin[(long long)m * g.ldz + n]));
                     g.Yout[(long long)m * g.ldy + n] = y;
-                    if (g.YTout)
-                        g.YTout[(long long)n * g.ldt + m] = y;
+                    if (stage_t)
+                        Tt[nl][ml] = y;
                 } else {
                     g.Cout[(long long)blockIdx.z * g.cstride + (long long)m * g.ldc + n] = v;
                 }
             }
+    if (stage_t) {  // uniform across the workgroup
+        __syncthreads();
+        const bool vec = (g.ldt % 8) == 0 && ((uintptr_t)g.YTout & 15) == 0;
+#pragma unroll
+        for (int r = 0; r < TN * TM / 8 / 256; ++r) {
+            const int slot = (int)threadIdx.x + 256 * r;
+            const int nl = slot / (TM / 8), ml = (slot % (TM / 8)) * 8;
+            const int n = n0 + nl, m = m0 + ml;
+            if (n >= g.N)
+                continue;
+            bf16_t *dst = g.YTout + (long long)n * g.ldt + m;
+            if (vec && m + 8 <= g.M) {
+                *(u16x8 *)dst = *(const u16x8 *)&Tt[nl][ml];
+            } else {
+                for (int e = 0; e < 8 && m + e < g.M; ++e)
+                    dst[e] = Tt[nl][ml + e];
+            }
+        }
+    }
 }
 
 // ---------------------------------------------------------------- encode
