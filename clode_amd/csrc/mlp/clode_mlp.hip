@@ -67,12 +67,19 @@ __host__ __device__ inline float bf2f(bf16_t h)
     return f;
 }
 
-__device__ inline float silu(float z) { return z / (1.0f + __expf(-z)); }
+// logistic sigmoid with the hardware exponential and reciprocal (v_exp_f32,
+// v_rcp_f32; about 1 ulp each): the GEMM epilogues evaluate it once per
+// output element, and an IEEE division there cost more than the products
+__device__ inline float sigm(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ inline float silu(float z) { return z * sigm(z); }
 __device__ inline float dsilu(float z)
 {
-    float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigm(z);
     return s * (1.0f + z * (1.0f - s));
 }
+// float -> BF16 on the device, round to nearest even: the gfx950 conversion
+// instruction, which agrees with f2bf for every finite input
+__device__ inline bf16_t f2bf_dev(float f) { return __builtin_bit_cast(bf16_t, (__bf16)f); }
 
 // ---------------------------------------------------------------- GEMM
 // C[M x N] = A[M x K] * B[K x N] with arbitrary strides on both operands,
@@ -103,20 +110,36 @@ struct GemmArgs {
     long long cstride;  // Cout offset between splits (EPI_WGRAD)
 };
 
-constexpr int TM = 64, TN = 64, TK = 32, KC = 128, KPAD = 8;
-constexpr int KCP = KC + KPAD;
-constexpr int KG = KC / 8;                 // 16-byte groups per staged row
-constexpr int LD_PER_T = TM * KG / 256;    // 16-byte loads per thread per operand and chunk
+constexpr int TK = 32, KPAD = 8;
 
-// rows [r0, r0 + 64) x k [k, k + KC) of a strided BF16 operand into registers
+// Workgroup tiles: 64 x 64 (each of the 4 waves 32 x 32 = 2 x 2 MFMA blocks,
+// K staged 128 deep) for the training products of a few thousand rows, where
+// the number of workgroups matters; 128 x 128 (each wave 64 x 64 = 4 x 4
+// blocks, K staged 64 deep) for the screening products, where every 16-byte
+// LDS operand read then feeds four MFMAs instead of two.  The row pitch of
+// KC + 8 elements keeps the 16 rows an operand read touches on disjoint banks.
+template <int T>
+struct Tile;
+template <>
+struct Tile<64> {
+    static constexpr int TM = 64, TN = 64, KC = 128, WB = 2;
+};
+template <>
+struct Tile<128> {
+    static constexpr int TM = 128, TN = 128, KC = 64, WB = 4;
+};
+
+// rows [r0, r0 + TM) x k [k, k + KC) of a strided BF16 operand into registers
 // (zero outside [0, rows) x [k, kend)); slot j of thread t is row
-// (t + 256 j) / KG, group (t + 256 j) % KG, so 16 consecutive threads read one
-// contiguous 256-byte row segment
+// (t + 256 j) / KG, group (t + 256 j) % KG, so consecutive threads read one
+// contiguous row segment
+template <int TM, int KC>
 __device__ inline void gemm_load(const bf16_t *X, long long ld_r, long long ld_k, bool vec, int rows, int r0,
-                                 int k, int kend, u16x8 (&v)[LD_PER_T])
+                                 int k, int kend, u16x8 (&v)[TM * (KC / 8) / 256])
 {
+    constexpr int KG = KC / 8;
 #pragma unroll
-    for (int j = 0; j < LD_PER_T; ++j) {
+    for (int j = 0; j < TM * KG / 256; ++j) {
         const int slot = (int)threadIdx.x + 256 * j;
         const int r = r0 + slot / KG, kk = k + (slot % KG) * 8;
         if (vec && r < rows && kk + 8 <= kend) {
@@ -129,60 +152,66 @@ __device__ inline void gemm_load(const bf16_t *X, long long ld_r, long long ld_k
     }
 }
 
-__device__ inline void gemm_stage(bf16_t (*S)[KCP], const u16x8 (&v)[LD_PER_T])
+template <int TM, int KC>
+__device__ inline void gemm_stage(bf16_t *S, const u16x8 (&v)[TM * (KC / 8) / 256])
 {
+    constexpr int KG = KC / 8, KCP = KC + KPAD;
 #pragma unroll
-    for (int j = 0; j < LD_PER_T; ++j) {
+    for (int j = 0; j < TM * KG / 256; ++j) {
         const int slot = (int)threadIdx.x + 256 * j;
-        *(u16x8 *)&S[slot / KG][(slot % KG) * 8] = v[j];
+        *(u16x8 *)&S[(slot / KG) * KCP + (slot % KG) * 8] = v[j];
     }
 }
 
-template <int EPI>
+template <int EPI, int T>
 __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
 {
-    __shared__ __attribute__((aligned(16))) bf16_t As[TM][KCP];
-    __shared__ __attribute__((aligned(16))) bf16_t Bs[TN][KCP];
+    constexpr int TM = Tile<T>::TM, TN = Tile<T>::TN, KC = Tile<T>::KC, WB = Tile<T>::WB;
+    constexpr int KCP = KC + KPAD, LDT = TM * (KC / 8) / 256;
+    __shared__ __attribute__((aligned(16))) bf16_t smem[(TM + TN) * KCP];
+    bf16_t *As = smem, *Bs = smem + TM * KCP;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
     const int kbeg = g.ksplit > 0 ? (int)blockIdx.z * g.ksplit : 0;
     const int kend = g.ksplit > 0 ? min(g.K, kbeg + g.ksplit) : g.K;
-    f32x4 acc[2][2];
+    f32x4 acc[WB][WB];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < WB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < WB; ++j)
             acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
     const bool a_vec = g.lda_k == 1 && (g.lda_m % 8) == 0 && ((uintptr_t)g.A & 15) == 0;
     const bool b_vec = g.ldb_k == 1 && (g.ldb_n % 8) == 0 && ((uintptr_t)g.B & 15) == 0;
-    u16x8 va[LD_PER_T], vb[LD_PER_T];
+    u16x8 va[LDT], vb[LDT];
     if (kbeg < kend) {
-        gemm_load(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kbeg, kend, va);
-        gemm_load(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kbeg, kend, vb);
+        gemm_load<TM, KC>(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kbeg, kend, va);
+        gemm_load<TN, KC>(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kbeg, kend, vb);
     }
     for (int kc = kbeg; kc < kend; kc += KC) {
-        gemm_stage(As, va);
-        gemm_stage(Bs, vb);
+        gemm_stage<TM, KC>(As, va);
+        gemm_stage<TN, KC>(Bs, vb);
         __syncthreads();
         if (kc + KC < kend) {  // next chunk in flight during this chunk's MFMAs
-            gemm_load(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kc + KC, kend, va);
-            gemm_load(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kc + KC, kend, vb);
+            gemm_load<TM, KC>(g.A, g.lda_m, g.lda_k, a_vec, g.M, m0, kc + KC, kend, va);
+            gemm_load<TN, KC>(g.B, g.ldb_n, g.ldb_k, b_vec, g.N, n0, kc + KC, kend, vb);
         }
         const int nks = (min(KC, kend - kc) + TK - 1) / TK;
         const int r = lane & 15;
         for (int ks = 0; ks < nks; ++ks) {
             const int kk = ks * TK + (lane >> 4) * 8;
+            bf16x8 bfr[WB];
 #pragma unroll
-            for (int i = 0; i < 2; ++i) {
-                bf16x8 a = *(const bf16x8 *)&As[wm * 32 + i * 16 + r][kk];
+            for (int j = 0; j < WB; ++j)
+                bfr[j] = *(const bf16x8 *)&Bs[(wn * (TN / 2) + j * 16 + r) * KCP + kk];
 #pragma unroll
-                for (int j = 0; j < 2; ++j) {
-                    bf16x8 b = *(const bf16x8 *)&Bs[wn * 32 + j * 16 + r][kk];
-                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
-                }
+            for (int i = 0; i < WB; ++i) {
+                const bf16x8 a = *(const bf16x8 *)&As[(wm * (TM / 2) + i * 16 + r) * KCP + kk];
+#pragma unroll
+                for (int j = 0; j < WB; ++j)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, bfr[j], acc[i][j], 0, 0, 0);
             }
         }
         __syncthreads();  // before the next chunk overwrites the tiles
@@ -193,17 +222,17 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
     // written a row of YT at a time with 16-byte stores, instead of one
     // scattered 2-byte store per element.
     constexpr int TP = TM + 8;  // pitch of the transposed tile
-    static_assert(TN * TP <= TM * KCP, "the transposed tile fits in the A tile's LDS");
-    bf16_t(*Tt)[TP] = reinterpret_cast<bf16_t(*)[TP]>(&As[0][0]);
+    static_assert(TN * TP <= (TM + TN) * KCP, "the transposed tile fits in the staging LDS");
+    bf16_t *Tt = smem;
     const bool stage_t = (EPI == EPI_FWD_SILU || EPI == EPI_BWD_DSILU) && g.YTout != nullptr;
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < WB; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < WB; ++j)
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const int ml = wm * 32 + i * 16 + (lane >> 4) * 4 + q;
-                const int nl = wn * 32 + j * 16 + (lane & 15);
+                const int ml = wm * (TM / 2) + i * 16 + (lane >> 4) * 4 + q;
+                const int nl = wn * (TN / 2) + j * 16 + (lane & 15);
                 const int m = m0 + ml;
                 const int n = n0 + nl;
                 if (m >= g.M || n >= g.N)
@@ -214,16 +243,16 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
                     if (EPI == EPI_FWD_LINEAR || g.Zout)
                         g.Zout[(long long)m * g.ldz + n] = v;
                     if (EPI == EPI_FWD_SILU) {
-                        const bf16_t y = f2bf(silu(v));
+                        const bf16_t y = f2bf_dev(silu(v));
                         g.Yout[(long long)m * g.ldy + n] = y;
                         if (stage_t)
-                            Tt[nl][ml] = y;
+                            Tt[nl * TP + ml] = y;
                     }
                 } else if (EPI == EPI_BWD_DSILU) {
-                    const bf16_t y = f2bf(v * dsilu(g.Zin[(long long)m * g.ldz + n]));
+                    const bf16_t y = f2bf_dev(v * dsilu(g.Zin[(long long)m * g.ldz + n]));
                     g.Yout[(long long)m * g.ldy + n] = y;
                     if (stage_t)
-                        Tt[nl][ml] = y;
+                        Tt[nl * TP + ml] = y;
                 } else {
                     g.Cout[(long long)blockIdx.z * g.cstride + (long long)m * g.ldc + n] = v;
                 }
@@ -232,18 +261,18 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
         __syncthreads();
         const bool vec = (g.ldt % 8) == 0 && ((uintptr_t)g.YTout & 15) == 0;
 #pragma unroll
-        for (int r = 0; r < TN * TM / 8 / 256; ++r) {
-            const int slot = (int)threadIdx.x + 256 * r;
+        for (int rr = 0; rr < TN * TM / 8 / 256; ++rr) {
+            const int slot = (int)threadIdx.x + 256 * rr;
             const int nl = slot / (TM / 8), ml = (slot % (TM / 8)) * 8;
             const int n = n0 + nl, m = m0 + ml;
             if (n >= g.N)
                 continue;
             bf16_t *dst = g.YTout + (long long)n * g.ldt + m;
             if (vec && m + 8 <= g.M) {
-                *(u16x8 *)dst = *(const u16x8 *)&Tt[nl][ml];
+                *(u16x8 *)dst = *(const u16x8 *)&Tt[nl * TP + ml];
             } else {
                 for (int e = 0; e < 8 && m + e < g.M; ++e)
-                    dst[e] = Tt[nl][ml + e];
+                    dst[e] = Tt[nl * TP + ml + e];
             }
         }
     }
@@ -627,17 +656,29 @@ struct clode_mlp {
 
 namespace {
 
-void gemm(clode_mlp *h, int epi, const GemmArgs &a)
+template <int T>
+void gemm_tiled(clode_mlp *h, int epi, const GemmArgs &a)
 {
+    constexpr int TM = Tile<T>::TM, TN = Tile<T>::TN;
     const int splits = a.ksplit > 0 ? (a.K + a.ksplit - 1) / a.ksplit : 1;
     dim3 grid((a.N + TN - 1) / TN, (a.M + TM - 1) / TM, splits);
     switch (epi) {
-    case EPI_FWD_SILU: hipLaunchKernelGGL(mlp_gemm<EPI_FWD_SILU>, grid, dim3(256), 0, h->stream, a); break;
-    case EPI_FWD_LINEAR: hipLaunchKernelGGL(mlp_gemm<EPI_FWD_LINEAR>, grid, dim3(256), 0, h->stream, a); break;
-    case EPI_BWD_DSILU: hipLaunchKernelGGL(mlp_gemm<EPI_BWD_DSILU>, grid, dim3(256), 0, h->stream, a); break;
-    default: hipLaunchKernelGGL(mlp_gemm<EPI_WGRAD>, grid, dim3(256), 0, h->stream, a); break;
+    case EPI_FWD_SILU: hipLaunchKernelGGL((mlp_gemm<EPI_FWD_SILU, T>), grid, dim3(256), 0, h->stream, a); break;
+    case EPI_FWD_LINEAR: hipLaunchKernelGGL((mlp_gemm<EPI_FWD_LINEAR, T>), grid, dim3(256), 0, h->stream, a); break;
+    case EPI_BWD_DSILU: hipLaunchKernelGGL((mlp_gemm<EPI_BWD_DSILU, T>), grid, dim3(256), 0, h->stream, a); break;
+    default: hipLaunchKernelGGL((mlp_gemm<EPI_WGRAD, T>), grid, dim3(256), 0, h->stream, a); break;
     }
     check(hipGetLastError(), "mlp_gemm launch");
+}
+
+// 128 x 128 tiles once the product has enough rows to keep every CU busy
+// with them (screening chunks); 64 x 64 below that (training minibatches)
+void gemm(clode_mlp *h, int epi, const GemmArgs &a)
+{
+    if (a.ksplit == 0 && a.M >= 32768)
+        gemm_tiled<128>(h, epi, a);
+    else
+        gemm_tiled<64>(h, epi, a);
 }
 
 GemmArgs ga() { GemmArgs a; std::memset(&a, 0, sizeof(a)); return a; }

@@ -44,6 +44,23 @@ def test_forward_matches_oracle(gpu_available):
     assert np.median(err) < 1e-4
 
 
+def test_forward_large_tiles(gpu_available):
+    """Products of 32,768 rows or more use 128 x 128 workgroup tiles; each
+    output element still accumulates the same K steps in the same order, so
+    the result equals the 64 x 64-tile result bit for bit (40,000 rows at
+    once against two halves of 20,000), and the oracle on a sample."""
+    m = net(4)
+    X = np.random.default_rng(2).random((40000, 2)).astype(np.float32)
+    big, enc = m.forward(X, with_encoding=True)
+    halves = np.concatenate([m.forward(X[:20000]), m.forward(X[20000:])])
+    assert np.array_equal(big, halves)
+    idx = np.arange(0, 40000, 97)
+    ref, _ = mo.forward(m.get_params().astype(np.float64), mo.bf16_bits_to_float(enc[idx]))
+    err = np.abs(big[idx] - ref[:, :mo.NOUT])
+    assert np.max(err / (1 + np.abs(ref[:, :mo.NOUT]))) < 1e-2
+    assert np.median(err) < 1e-4
+
+
 def test_gradient_matches_oracle(gpu_available):
     m = net(5)
     rng = np.random.default_rng(1)
