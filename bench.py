@@ -424,7 +424,9 @@ def surrogate_c5(precision, n=2048):
                                  single_precision=precision, dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6,
                                  max_steps=10 ** 8, observer_x_up_thresh=0.3, observer_x_down_thresh=0.2)
     solve = integrator_solver(sim, ("gk", "gbk"), ((0.0, 10.0), (0.0, 2.0)), 5000.0, 20000.0, "v")
-    solve(np.random.default_rng(0).random((4096, 2)))  # compile and first launch, untimed
+    # compile and first launch, untimed; 2^18 points so that the variant used
+    # for large ensembles (compiled on first use) is built here too
+    solve(np.random.default_rng(0).random((1 << 18, 2)))
     gx, gy = np.meshgrid(np.arange(n) / (n - 1), np.arange(n) / (n - 1), indexing="ij")
     t0 = time.perf_counter()
     plain = solve(np.stack([gx.ravel(), gy.ravel()], axis=1))

@@ -88,6 +88,7 @@ struct ProgramConfig {
     int observer = 0;  // 0 = none
     int eVar = 0, fVar = 0;  // observer event / feature variable
     int nVar = 0, nPar = 0, nAux = 0, nWiener = 0, nStore = 0;
+    int wavesPerEU = 0;  // > 0: kernels compiled for at least this many waves per SIMD
     std::string rhs;
     std::string rhs_name;
 };

@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <fstream>
 #include <random>
@@ -149,6 +150,46 @@ std::shared_ptr<Program> Simulator::currentProgram()
         programKeyCache = key;
     }
     return program;
+}
+
+std::shared_ptr<Program> Simulator::occupancyProgram()
+{
+    ProgramConfig c = programConfig();
+    c.wavesPerEU = 2;
+    std::string key = compose_source(c);
+    if (!programW2 || key != programW2Key) {
+        programW2 = get_program(c, root.empty() ? std::string() : root);
+        programW2Key = key;
+    }
+    return programW2;
+}
+
+// Kernel for a grid launch of nThreads members.  A kernel whose register
+// footprint allows one wave per SIMD leaves each SIMD without a second wave
+// to switch to while its one wave waits; when the ensemble has at least four
+// waves of members per SIMD, the variant compiled for two waves per SIMD
+// (registers capped at 256, the rest spilled) is used instead.  Both
+// variants perform the same arithmetic in the same order.
+// CLODE_OCCUPANCY_VARIANT=0 in the environment keeps the default kernel.
+hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads, unsigned int block)
+{
+    hipFunction_t fn = currentProgram()->function(device, kernel);
+    static const bool enabled = [] {
+        const char *e = std::getenv("CLODE_OCCUPANCY_VARIANT");
+        return !(e && std::strcmp(e, "0") == 0);
+    }();
+    if (!enabled || block % 64 != 0)
+        return fn;
+    int cus = 0, perCU = 0;
+    hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
+    const size_t simds = (size_t)std::max(cus, 1) * 4;
+    if ((size_t)nThreads < simds * 64 * 4)
+        return fn;
+    hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, (int)block, 0),
+              "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
+    if ((size_t)perCU * block / 64 > 4)  // more than one wave per SIMD already
+        return fn;
+    return occupancyProgram()->function(device, kernel);
 }
 
 // CLODE::buildCL (CLODE.cpp:155-171) + the subclasses' kernel creation
@@ -490,7 +531,7 @@ void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThr
         lastKernelMs = ms;
         return;
     }
-    hipFunction_t fn = p->function(device, kernel);
+    hipFunction_t fn = gridFunction(kernel, nThreads, block);
     unsigned int grid = (nThreads + block - 1) / block;
     hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
     hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr),

@@ -119,6 +119,12 @@ private:
     int observerId() const;
     ProgramConfig programConfig() const;
     std::shared_ptr<Program> currentProgram();
+    // the same program compiled for two waves per SIMD, used by grid launches
+    // of large ensembles whose default kernel fits only one wave per SIMD
+    std::shared_ptr<Program> occupancyProgram();
+    std::shared_ptr<Program> programW2;
+    std::string programW2Key;
+    hipFunction_t gridFunction(const char *kernel, unsigned int nThreads, unsigned int block);
     void ensureStream();
     void setNpts(int n);
     void upload(DeviceBuffer &b, const std::vector<double> &v, size_t elem);

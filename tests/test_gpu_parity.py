@@ -8,11 +8,12 @@ not merely close.  Ensemble sizes are deliberately not multiples of the
 64-lane wavefront.
 """
 import math
+import os
 
 import numpy as np
 import pytest
 
-from helpers import Pair
+from helpers import MODEL_INFO, MODELS, OBS_DEFAULTS, Pair, wrapper
 
 pytestmark = pytest.mark.gpu
 
@@ -242,3 +243,32 @@ def test_step_counter_matches_oracle(gpu_available):
     p.features()
     total = np.array(p.gpu.get_step_counts(), dtype=np.int64).sum()
     assert total == p.orc.last_steps + p.orc.last_warm_steps
+
+
+def test_large_ensemble_variant_bitexact(gpu_available):
+    """Ensembles of at least four waves per SIMD run the kernels compiled for
+    two waves per SIMD when the default ones fit only one (lactotroph
+    features in FP64): same arithmetic, so the features of 262,144 members
+    equal those of the same members run as two ensembles of 131,072."""
+    w = wrapper()
+    vars_, pars, aux, noise = MODEL_INFO["lactotroph"]
+    pi = w.ProblemInfo(os.path.join(MODELS, "lactotroph.cl"), vars_, pars, aux, noise)
+    op = w.ObserverParams(**OBS_DEFAULTS)
+
+    def run(x0, p):
+        s = w.FeatureSimulatorBase(pi, "dopri5", "thresh2", op, "double", w.OpenCLResource(), "")
+        s.build_cl()
+        s.set_solver_params(w.SolverParams(0.1, 1.0, 1e-8, 1e-6, 1000000, 1000000, 1))
+        s.set_problem_data(np.ascontiguousarray(x0).ravel(), np.ascontiguousarray(p).ravel())
+        s.set_tspan([0.0, 300.0])
+        s.features(True)
+        return np.array(s.get_f()).reshape(-1, x0.shape[1]), np.array(s.get_xf())
+
+    n = 1 << 18
+    x0, p = lac_data(n)
+    F, xf = run(x0, p)
+    h = n // 2
+    F1, xf1 = run(x0[:, :h], p[:, :h])
+    F2, xf2 = run(x0[:, h:], p[:, h:])
+    assert np.array_equal(F, np.concatenate([F1, F2], axis=1))
+    assert np.array_equal(xf.reshape(4, n), np.concatenate([xf1.reshape(4, h), xf2.reshape(4, h)], axis=1))
