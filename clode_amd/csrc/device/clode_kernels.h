@@ -198,8 +198,8 @@ CLODE_DEV void member_tspan(const ClodeArgs &a, unsigned int i, timetype &t0, ti
 
 #if CLODE_PREC == 3
 #define CLODE_STEP(m)                                                                              \
-    step_attempt(m.ti, m.xi, m.xc, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd, a.sp_reltol,       \
-                 a.sp_abstol / a.sp_reltol, a.sp_dtmax, m.ctl)
+    step_attempt(m.ti, m.xi, m.xc, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd, a.sp_reltol,           \
+                 (realtype)(a.sp_abstol / a.sp_reltol), a.sp_dtmax, m.ctl)
 #else
 #define CLODE_STEP(m) step_attempt(m.ti, m.xi, m.dxi, m.p, sp, m.dt, t1, m.auxi, m.wi, m.rd, m.ctl)
 #endif

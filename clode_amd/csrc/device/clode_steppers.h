@@ -21,9 +21,12 @@
 //
 // fp32-compensated mode (CLODE_PREC 3) keeps the same tableaux but carries
 // time and the step size in double, adds each state increment with Kahan
-// compensation (x, c), evaluates the embedded error combination in double
-// (full-rate FP64 FMA on CDNA) and drives the controller from that double
-// error norm; stage algebra and the right-hand side run in float.
+// compensation (x, c) and drives the controller from a double error norm
+// and a double accept/reject test; stage algebra, the right-hand side and
+// the embedded error combination h sum(E_i k_i) run in float.  That
+// combination rounds to about one float ulp of h sum(|E_i k_i|), the same
+// size as the rounding the float stage derivatives k_i already carry, so
+// forming it in double would not sharpen the estimate.
 #ifndef CLODE_STEPPERS_H
 #define CLODE_STEPPERS_H
 
@@ -332,7 +335,7 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
 #else  // CLODE_PREC == 3
 // =====================================================================
 //  fp32-compensated: float state + Kahan compensation, double time/dt,
-//  double embedded-error norm
+//  float embedded-error estimate, double error norm
 // =====================================================================
 
 // x + d with Kahan compensation c (c holds the negated lost low part)
@@ -418,12 +421,12 @@ CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k
 #define CLODE_MAX_SHRINK 0.5
 #define CLODE_MAX_GROW 5.0
 // returns the step taken; newxi holds the 3rd-order solution without the
-// compensation term; errd the double error estimate
+// compensation term; err the error estimate
 CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtype pars[],
-                         const double dt, realtype aux[], double errd[], realtype dx[], const realtype wi[])
+                         const double dt, realtype aux[], realtype err[], realtype dx[], const realtype wi[])
 {
     const realtype B1 = 2.0f / 9.0f, B2 = 1.0f / 3.0f, B3 = 4.0f / 9.0f;
-    const double E1 = -5.0 / 72.0, E2 = 1.0 / 12.0, E3 = 1.0 / 9.0, E4 = -1.0 / 8.0;
+    const realtype E1 = -5.0f / 72.0f, E2 = 1.0f / 12.0f, E3 = 1.0f / 9.0f, E4 = -1.0f / 8.0f;
     double tNew = ti + dt;
     double newDt = tNew - ti;
     const realtype h = (realtype)newDt;
@@ -440,7 +443,7 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
     }
     getRHS((realtype)tNew, xtmp, pars, k4, aux, wi);
     for (int k = 0; k < N_VAR; k++) {
-        errd[k] = newDt * (E1 * (double)k1[k] + E2 * (double)k2[k] + E3 * (double)k3[k] + E4 * (double)k4[k]);
+        err[k] = h * (E1 * k1[k] + E2 * k2[k] + E3 * k3[k] + E4 * k4[k]);
         k1[k] = k4[k];
         xi[k] = xtmp[k];
     }
@@ -454,7 +457,7 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
 #define CLODE_MAX_SHRINK 0.1
 #define CLODE_MAX_GROW 5.0
 CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtype pars[],
-                         const double dt, realtype aux[], double errd[], realtype dx[], const realtype wi[])
+                         const double dt, realtype aux[], realtype err[], realtype dx[], const realtype wi[])
 {
     const realtype B21 = 1.0f / 5.0f;
     const realtype B31 = 3.0f / 40.0f, B32 = 9.0f / 40.0f;
@@ -465,8 +468,8 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
                    B64 = 49.0f / 176.0f, B65 = -5103.0f / 18656.0f;
     const realtype C1 = 35.0f / 384.0f, C3 = 500.0f / 1113.0f, C4 = 125.0f / 192.0f,
                    C5 = -2187.0f / 6784.0f, C6 = 11.0f / 84.0f;
-    const double E1 = 71.0 / 57600.0, E3 = -71.0 / 16695.0, E4 = 71.0 / 1920.0,
-                 E5 = -17253.0 / 339200.0, E6 = 22.0 / 525.0, E7 = -1.0 / 40.0;
+    const realtype E1 = 71.0f / 57600.0f, E3 = -71.0f / 16695.0f, E4 = 71.0f / 1920.0f,
+                   E5 = -17253.0f / 339200.0f, E6 = 22.0f / 525.0f, E7 = -1.0f / 40.0f;
     double tNew = ti + dt;
     double newDt = tNew - ti;
     const realtype h = (realtype)newDt;
@@ -492,8 +495,7 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
     }
     getRHS((realtype)tNew, xtmp, pars, k7, aux, wi);
     for (int k = 0; k < N_VAR; k++) {
-        errd[k] = newDt * (E1 * (double)k1[k] + E3 * (double)k3[k] + E4 * (double)k4[k] +
-                           E5 * (double)k5[k] + E6 * (double)k6[k] + E7 * (double)k7[k]);
+        err[k] = h * (E1 * k1[k] + E3 * k3[k] + E4 * k4[k] + E5 * k5[k] + E6 * k6[k] + E7 * k7[k]);
         k1[k] = k7[k];
         xi[k] = xtmp[k];
     }
@@ -503,16 +505,18 @@ CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtyp
 #endif
 
 #if CLODE_ADAPTIVE_STEP
-// Same controller as the plain modes, one trial per call; the error ratio
-// and its norm are formed in double, the growth factor (rtol/err)^(1/(p+1))
-// with the hardware log2/exp2 (it only steers the next step size).
+// Same controller as the plain modes, one trial per call.  The error norm
+// max_j |err_j| / max(|x_j|, |x_j'|, atol/rtol) and its test against rtol
+// are double; the growth factor (rtol/err)^(1/(p+1)) is formed in float with
+// the hardware reciprocal, log2 and exp2 (it only steers the next step size,
+// which stays in double).
 CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k1[], const realtype pars[],
                             const SolverParamsR &sp, double &dt, const double t1,
-                            realtype aux[], realtype wi[], RngData &rd, const double reltol, const double threshold,
-                            const double dtmax, StepCtl &c)
+                            realtype aux[], realtype wi[], RngData &rd, const double reltol,
+                            const realtype threshold, const double dtmax, StepCtl &c)
 {
-    double tNew, normErr, errd[N_VAR];
-    realtype newxi[N_VAR], newk1[N_VAR], dx[N_VAR];
+    double tNew, normErr;
+    realtype err[N_VAR], newxi[N_VAR], newk1[N_VAR], dx[N_VAR];
     double hmin = 16.0 * fabs(fabs(nextafter(ti, 1.1 * t1)) - ti);
     if (!c.inStep) {
         c.inStep = true;
@@ -531,15 +535,18 @@ CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k
         newk1[j] = k1[j];
     }
     double newDt = fmin(fmax(c.newDt, hmin), dtmax);
-    newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, errd, dx, wi);
+    newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, dx, wi);
     normErr = 0.0;
     for (int j = 0; j < N_VAR; j++) {
-        realtype sc = fmaxf(fmaxf(fabsf(xi[j]), fabsf(newxi[j])), (realtype)threshold);
-        normErr = fmax(fabs(errd[j]) * (double)__frcp_rn(sc), normErr);
+        realtype sc = fmaxf(fmaxf(fabsf(xi[j]), fabsf(newxi[j])), threshold);
+        normErr = fmax((double)fabsf(err[j]) * (double)__frcp_rn(sc), normErr);
     }
-    // the factor both outcomes use (a rejected trial has normErr > reltol > 0)
-    const realtype q = normErr > 0.0 ? (realtype)(reltol / normErr) : 3.0e38f;
-    const double factor = (double)(0.8f * exp2f(CLODE_EXPON * log2f(q)));
+    // the factor both outcomes use (a rejected trial has normErr > reltol > 0);
+    // q is clamped to normal floats so that the hardware log2 sees no
+    // denormal, zero or infinity
+    const realtype q =
+        fminf(fmaxf((realtype)reltol * __builtin_amdgcn_rcpf((realtype)normErr), 1.0e-30f), 1.0e30f);
+    const double factor = (double)(0.8f * __builtin_amdgcn_exp2f(CLODE_EXPON * __builtin_amdgcn_logf(q)));
     if (normErr > reltol) {
         if (newDt <= hmin) {
             dt = hmin;
