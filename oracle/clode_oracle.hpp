@@ -2,8 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  This is a plain restatement of the reference's
 // algorithm, used by tests/, __graft_entry__.smoke() and bench.py's
-// cpu_baseline leg as the checker (and as the timed CPU baseline).  The
-// library never includes, links or calls it.
+// cpu_baseline leg to verify GPU results (and as the timed CPU baseline).
+// The library never includes, links or calls it.
 //
 // It follows the reference's own data layout and control flow rather than the
 // GPU design: full ObserverData structs with 3-point buffers for every

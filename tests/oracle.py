@@ -1,7 +1,7 @@
 """ctypes front end of the CPU oracle (oracle/oracle_model.cpp).
 
 Test infrastructure: used by the tests, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg as the checker.  ``OracleEnsemble`` mirrors the reference's
+cpu_baseline leg to verify GPU results.  ``OracleEnsemble`` mirrors the reference's
 host API (set_problem_data, set_tspan, set_solver_params, seed_rng,
 transient, features, trajectory, shift_x0, getters), so a test drives the GPU
 simulator and the oracle with the same call sequence.

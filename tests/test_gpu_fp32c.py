@@ -1,7 +1,7 @@
 """fp32-compensated mode against the FP64 CPU oracle.
 
-Stated tolerances (north_star: "adaptive FP32 states within a stated rtol",
-"each feature's error against an FP64 oracle is stated"):
+Stated tolerances (adaptive FP32 states within a stated rtol; each feature's
+error against the FP64 oracle stated):
   * fixed-step rk4 Van der Pol over 2,000 steps: |x - x_fp64| <= 2e-5 (1 + |x_fp64|);
   * fixed-step Euler on decay over 1e5 steps: the Kahan-compensated state
     error against FP64 is at least 10x smaller than plain single precision;

@@ -4,7 +4,7 @@ tests/sanitize/oracle_driver.cpp runs every kind (base, features,
 trajectory) x stepper x observer x precision, with max_store 0, 1 and 7,
 observer event lists, and per-member time spans, on two models (a plain ODE
 and a stochastic one).  Any out-of-bounds access or undefined operation in
-the checker fails this test.  Host code only (clang++ from ROCm)."""
+the oracle fails this test.  Host code only (clang++ from ROCm)."""
 import os
 import shutil
 import subprocess
