@@ -245,18 +245,20 @@ def test_step_counter_matches_oracle(gpu_available):
     assert total == p.orc.last_steps + p.orc.last_warm_steps
 
 
-def test_large_ensemble_variant_bitexact(gpu_available):
-    """Ensembles of at least four waves per SIMD run the kernels compiled for
+@pytest.mark.parametrize("prec", ["double", "fp32c"])
+def test_large_ensemble_variant_bitexact(gpu_available, prec):
+    """Ensembles of at least two waves per SIMD run the kernels compiled for
     two waves per SIMD when the default ones fit only one (lactotroph
-    features in FP64): same arithmetic, so the features of 262,144 members
-    equal those of the same members run as two ensembles of 131,072."""
+    features in FP64 and fp32c): same arithmetic, so the features of 262,144
+    members equal those of the same members run as four ensembles of 65,536
+    (one wave per SIMD: the default kernels)."""
     w = wrapper()
     vars_, pars, aux, noise = MODEL_INFO["lactotroph"]
     pi = w.ProblemInfo(os.path.join(MODELS, "lactotroph.cl"), vars_, pars, aux, noise)
     op = w.ObserverParams(**OBS_DEFAULTS)
 
     def run(x0, p):
-        s = w.FeatureSimulatorBase(pi, "dopri5", "thresh2", op, "double", w.OpenCLResource(), "")
+        s = w.FeatureSimulatorBase(pi, "dopri5", "thresh2", op, prec, w.OpenCLResource(), "")
         s.build_cl()
         s.set_solver_params(w.SolverParams(0.1, 1.0, 1e-8, 1e-6, 1000000, 1000000, 1))
         s.set_problem_data(np.ascontiguousarray(x0).ravel(), np.ascontiguousarray(p).ravel())
@@ -267,8 +269,7 @@ def test_large_ensemble_variant_bitexact(gpu_available):
     n = 1 << 18
     x0, p = lac_data(n)
     F, xf = run(x0, p)
-    h = n // 2
-    F1, xf1 = run(x0[:, :h], p[:, :h])
-    F2, xf2 = run(x0[:, h:], p[:, h:])
-    assert np.array_equal(F, np.concatenate([F1, F2], axis=1))
-    assert np.array_equal(xf.reshape(4, n), np.concatenate([xf1.reshape(4, h), xf2.reshape(4, h)], axis=1))
+    q = n // 4
+    parts = [run(x0[:, k * q:(k + 1) * q], p[:, k * q:(k + 1) * q]) for k in range(4)]
+    assert np.array_equal(F, np.concatenate([Fk for Fk, _ in parts], axis=1))
+    assert np.array_equal(xf.reshape(4, n), np.concatenate([xk.reshape(4, q) for _, xk in parts], axis=1))
