@@ -24,6 +24,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <cstdlib>
 #include <cstring>
 #include <random>
 #include <string>
@@ -279,12 +280,21 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
 }
 
 // ---------------------------------------------------------------- encode
+// Fourier feature f of the input (x0, x1): sin and cos of
+// 2 pi (B[0][f] x0 + B[1][f] x1), rounded to BF16
+__device__ __attribute__((always_inline)) inline void encode_pair(float x0, float x1, int f, const float *Bff, bf16_t &s, bf16_t &c)
+{
+    const float p = 6.283185307179586f * (Bff[f] * x0 + Bff[NFF + f] * x1);
+    s = f2bf(sinf(p));
+    c = f2bf(cosf(p));
+}
+
 // row i of the encoded inputs; A0t (optional): transposed copy [32][ldt]
 __device__ inline void encode_row(float x0, float x1, int i, const float *Bff, bf16_t *A0, bf16_t *A0t, int ldt)
 {
     for (int f = 0; f < NFF; ++f) {
-        const float p = 6.283185307179586f * (Bff[f] * x0 + Bff[NFF + f] * x1);
-        const bf16_t s = f2bf(sinf(p)), c = f2bf(cosf(p));
+        bf16_t s, c;
+        encode_pair(x0, x1, f, Bff, s, c);
         A0[(long long)i * D0 + 2 * f] = s;
         A0[(long long)i * D0 + 2 * f + 1] = c;
         if (A0t) {
@@ -304,15 +314,19 @@ __global__ void __launch_bounds__(256) mlp_encode(const float *X, int n, const f
 }
 
 // grid inputs: cell c = ix * ny + iy at (lo + (hi - lo) * i / (n - 1)), normalised to [0, 1]
+__device__ __attribute__((always_inline)) inline void grid_point(long long c, int nx, int ny, float &x0, float &x1)
+{
+    const int ix = (int)(c / ny), iy = (int)(c % ny);
+    x0 = nx > 1 ? (float)ix / (float)(nx - 1) : 0.f;
+    x1 = ny > 1 ? (float)iy / (float)(ny - 1) : 0.f;
+}
+
 __global__ void __launch_bounds__(256) mlp_grid_inputs(long long c0, int n, int nx, int ny, float *X)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
-    const long long c = c0 + i;
-    const int ix = (int)(c / ny), iy = (int)(c % ny);
-    X[2 * i] = nx > 1 ? (float)ix / (float)(nx - 1) : 0.f;
-    X[2 * i + 1] = ny > 1 ? (float)iy / (float)(ny - 1) : 0.f;
+    grid_point(c0 + i, nx, ny, X[2 * i], X[2 * i + 1]);
 }
 
 // ---------------------------------------------------------------- loss
@@ -480,19 +494,20 @@ __global__ void __launch_bounds__(256) mlp_gather(const unsigned int *perm, int 
 // class argmax; the ensemble prediction is the head mean (softmax mean for the
 // class).  flags bit 0: the heads disagree (relative spread above tol, or a
 // head's class differs from the ensemble class).
-__global__ void __launch_bounds__(256) mlp_predict_cells(const float *Zh, int n, const float *tmean,
-                                                         const float *tstd, float tol, float *period, float *amp,
-                                                         int *cls, uint8_t *flags)
+// z: the 72 head outputs of one cell
+__device__ __attribute__((always_inline)) inline void predict_row(const float *z, const float *tmean, const float *tstd, float tol, float &period,
+                                   float &amp, int &cls, uint8_t &flags)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n)
-        return;
-    const float *z = Zh + (long long)i * NOUTP;
+    // every loop has a constant trip count and is unrolled, so the arrays
+    // live in registers (the first-occurrence argmax keeps its best value
+    // instead of indexing prob[] at a run-time index)
     float P[NHEAD], A[NHEAD], prob[NCLS];
     int hc[NHEAD];
+#pragma unroll
     for (int c = 0; c < NCLS; ++c)
         prob[c] = 0.f;
     float pm = 0.f, am = 0.f;
+#pragma unroll
     for (int h = 0; h < NHEAD; ++h) {
         const float *zh = z + h * HW;
         P[h] = expf(zh[0] * tstd[0] + tmean[0]);
@@ -501,6 +516,7 @@ __global__ void __launch_bounds__(256) mlp_predict_cells(const float *Zh, int n,
         am += A[h];
         float mx = zh[2];
         int am_i = 0;
+#pragma unroll
         for (int c = 1; c < NCLS; ++c)
             if (zh[2 + c] > mx) {
                 mx = zh[2 + c];
@@ -508,27 +524,44 @@ __global__ void __launch_bounds__(256) mlp_predict_cells(const float *Zh, int n,
             }
         hc[h] = am_i;
         float se = 0.f;
+#pragma unroll
         for (int c = 0; c < NCLS; ++c)
             se += __expf(zh[2 + c] - mx);
+#pragma unroll
         for (int c = 0; c < NCLS; ++c)
             prob[c] += __expf(zh[2 + c] - mx) / se;
     }
     pm *= 1.f / NHEAD;
     am *= 1.f / NHEAD;
     int best = 0;
+    float pbest = prob[0];
+#pragma unroll
     for (int c = 1; c < NCLS; ++c)
-        if (prob[c] > prob[best])
+        if (prob[c] > pbest) {
+            pbest = prob[c];
             best = c;
+        }
     bool dis = false;
+#pragma unroll
     for (int h = 0; h < NHEAD; ++h) {
         dis |= fabsf(P[h] - pm) > tol * fabsf(pm);
         dis |= fabsf(A[h] - am) > tol * fabsf(am);
         dis |= hc[h] != best;
     }
-    period[i] = pm;
-    amp[i] = am;
-    cls[i] = best;
-    flags[i] = dis ? 1 : 0;
+    period = pm;
+    amp = am;
+    cls = best;
+    flags = dis ? 1 : 0;
+}
+
+__global__ void __launch_bounds__(256) mlp_predict_cells(const float *Zh, int n, const float *tmean,
+                                                         const float *tstd, float tol, float *period, float *amp,
+                                                         int *cls, uint8_t *flags)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    predict_row(Zh + (long long)i * NOUTP, tmean, tstd, tol, period[i], amp[i], cls[i], flags[i]);
 }
 
 // flags bit 1: a 4-neighbour on the nx x ny grid has another class
@@ -565,6 +598,170 @@ __global__ void __launch_bounds__(256) mlp_dilate(const uint8_t *in, int nx, int
 
 // ---------------------------------------------------------------- host
 thread_local std::string g_err;
+
+// ---------------------------------------------------------------- fused screening
+// One workgroup per block of SROWS grid cells runs the whole network:
+// encode, the four hidden layers, the heads and the per-cell prediction.
+// The activations stay in LDS between layers; only the weights (0.4 MB,
+// L2-resident) and the four per-cell results touch memory, where the
+// layer-by-layer path writes and re-reads 0.5 KB of activations per cell and
+// layer.  Each wave owns 64 output columns of a hidden layer (8 x 4 blocks of
+// 16 x 16) or 32 of the heads (8 x 2), takes its weight fragments straight
+// from global memory with the next K step's in flight, and reads the
+// activations from LDS.  Operand layout, K order (32-deep steps, ascending)
+// and epilogue arithmetic are those of mlp_gemm, so every output equals the
+// layer-by-layer path bit for bit.
+//
+// LDS: the activations [SROWS][H] BF16, 64 KiB, two workgroups per CU.  The
+// 16-byte chunk kc of row m is stored at chunk kc ^ (m & 15) of that row, so
+// the 16 rows an operand read touches fall on 16 different chunk positions
+// (no padding, which would not fit two workgroups).  After the last hidden
+// layer the same bytes hold the FP32 head outputs [SROWS][ZPITCH].
+constexpr int SROWS = 128;         // cells per workgroup
+constexpr int ZPITCH = NOUT + 1;   // row pitch of the head outputs (floats)
+static_assert(SROWS * ZPITCH * 4 <= SROWS * H * 2, "head outputs fit in the activation buffer");
+static_assert(SROWS == 8 * 16 && H == 4 * 64 && NOUTP == 4 * 32, "wave tiling");
+
+struct ScreenArgs {
+    long long ncell;
+    int nx, ny;
+    const float *Bff;
+    const bf16_t *W[NLAYER];  // Wt[out][in], BF16
+    const float *bias[NLAYER];
+    const float *tm;  // tmean[2], tstd[2]
+    float tol;
+    float *period, *amp;
+    int *cls;
+    uint8_t *flags;
+};
+
+// The lane index, recomputed where it is used: an opaque value keeps the
+// compiler from computing every layer's epilogue LDS addresses once at the
+// top of the kernel, where they would stay live across all the K loops and
+// spill (two v_mbcnt instructions per epilogue instead).
+__device__ __attribute__((always_inline)) inline int lane_here()
+{
+    int v;
+    asm volatile("v_mbcnt_lo_u32_b32 %0, -1, 0\n\tv_mbcnt_hi_u32_b32 %0, -1, %0" : "=v"(v));
+    return v;
+}
+
+// element (m, k) of the swizzled activation tile
+__device__ __attribute__((always_inline)) inline int sw(int m, int k) { return m * H + ((((k >> 3) ^ (m & 15))) << 3) + (k & 7); }
+
+// acc[i][j] += act[rows of block i][0, K) * W[columns of block j][0, K)^T for
+// the NB column blocks starting at column n0 (16 NB columns)
+template <int K, int NB>
+__device__ __attribute__((always_inline)) inline void screen_product(const bf16_t *act, const bf16_t *__restrict__ W, int n0, int lane,
+                                      f32x4 (&acc)[8][NB])
+{
+    constexpr int NKS = K / TK;
+    const int r = lane & 15, kq = (lane >> 4) * 8;
+#pragma unroll
+    for (int i = 0; i < 8; ++i)
+#pragma unroll
+        for (int j = 0; j < NB; ++j)
+            acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+    bf16x8 b[NB], bn[NB];
+#pragma unroll
+    for (int j = 0; j < NB; ++j)
+        b[j] = *(const bf16x8 *)(W + (long long)(n0 + j * 16 + r) * K + kq);
+#pragma unroll 1
+    for (int ks = 0; ks < NKS; ++ks) {
+        if (ks + 1 < NKS) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                bn[j] = *(const bf16x8 *)(W + (long long)(n0 + j * 16 + r) * K + (ks + 1) * TK + kq);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const bf16x8 a = *(const bf16x8 *)&act[sw(i * 16 + r, ks * TK + kq)];
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+        }
+        if (ks + 1 < NKS) {
+#pragma unroll
+            for (int j = 0; j < NB; ++j)
+                b[j] = bn[j];
+        }
+    }
+}
+
+// hidden layer in place: act <- SiLU(act W^T + bias), BF16
+template <int K>
+__device__ __attribute__((always_inline)) inline void screen_hidden(bf16_t *act, const bf16_t *__restrict__ W, const float *__restrict__ bias,
+                                     int wave, int lane)
+{
+    f32x4 acc[8][4];
+    screen_product<K, 4>(act, W, wave * 64, lane, acc);
+    __syncthreads();  // every wave has read the layer's input
+    lane = lane_here();
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int n = wave * 64 + j * 16 + (lane & 15);
+        const float bj = bias[n];
+#pragma unroll
+        for (int i = 0; i < 8; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                float v = acc[i][j][q];
+                v += bj;
+                act[sw(i * 16 + (lane >> 4) * 4 + q, n)] = f2bf_dev(silu(v));
+            }
+    }
+    __syncthreads();
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) mlp_screen_fused(const ScreenArgs s)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t act[SROWS * H];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const long long c0 = (long long)blockIdx.x * SROWS;
+    {  // encode: thread t -> cell row t / 2, frequencies (t % 2) * 8 .. + 8
+        const int m = t >> 1, f0 = (t & 1) * (NFF / 2);
+        const long long c = c0 + m < s.ncell ? c0 + m : s.ncell - 1;
+        float x0, x1;
+        grid_point(c, s.nx, s.ny, x0, x1);
+        for (int f = f0; f < f0 + NFF / 2; ++f) {
+            bf16_t sv, cv;
+            encode_pair(x0, x1, f, s.Bff, sv, cv);
+            act[sw(m, 2 * f)] = sv;
+            act[sw(m, 2 * f + 1)] = cv;
+        }
+    }
+    __syncthreads();
+    screen_hidden<D0>(act, s.W[0], s.bias[0], wave, lane);
+    screen_hidden<H>(act, s.W[1], s.bias[1], wave, lane);
+    screen_hidden<H>(act, s.W[2], s.bias[2], wave, lane);
+    screen_hidden<H>(act, s.W[3], s.bias[3], wave, lane);
+    // heads: 128 columns, 32 per wave; the 72 used ones go to LDS in FP32
+    f32x4 acc[8][2];
+    screen_product<H, 2>(act, s.W[NHID], wave * 32, lane, acc);
+    __syncthreads();  // the activations are no longer needed: z overlays them
+    float *z = reinterpret_cast<float *>(act);
+    const int hl = lane_here();
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+        const int n = wave * 32 + j * 16 + (hl & 15);
+        if (n < NOUT) {
+            const float bj = s.bias[NHID][n];
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    float v = acc[i][j][q];
+                    v += bj;
+                    z[(i * 16 + (hl >> 4) * 4 + q) * ZPITCH + n] = v;
+                }
+        }
+    }
+    __syncthreads();
+    if (t < SROWS && c0 + t < s.ncell) {
+        const long long c = c0 + t;
+        predict_row(z + t * ZPITCH, s.tm, s.tm + 2, s.tol, s.period[c], s.amp[c], s.cls[c], s.flags[c]);
+    }
+}
 
 struct MlpError : std::runtime_error {
     using std::runtime_error::runtime_error;
@@ -1117,8 +1314,13 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
         throw std::invalid_argument("dilation radius must be 0..64");
     check(hipSetDevice(h->device), "hipSetDevice");
     const long long ncell = (long long)nx * ny;
+    // the fused kernel by default; CLODE_MLP_FUSED_SCREEN=0 runs the network
+    // layer by layer over chunks of 2^18 cells (same results, bit for bit)
+    const char *fe = std::getenv("CLODE_MLP_FUSED_SCREEN");
+    const bool fused = !(fe && std::strcmp(fe, "0") == 0);
     const int chunk = (int)std::min<long long>(ncell, 1 << 18);
-    h->ensureRows((size_t)chunk);
+    if (!fused)
+        h->ensureRows((size_t)chunk);
     Dev<float> dP, dA;
     Dev<int> dC;
     Dev<uint8_t> dF, dF2;
@@ -1130,7 +1332,30 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
     float tm[4] = {h->tmean[0], h->tmean[1], h->tstd[0], h->tstd[1]};
     upload(h, h->dtm.p, tm, 16);
     check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
-    for (long long c0 = 0; c0 < ncell; c0 += chunk) {
+    if (fused) {
+        ScreenArgs a;
+        std::memset(&a, 0, sizeof(a));
+        a.ncell = ncell;
+        a.nx = nx;
+        a.ny = ny;
+        a.Bff = h->dBff.p;
+        for (int l = 0; l < NLAYER; ++l) {
+            a.W[l] = h->pbf.p + h->L[l].woff;
+            a.bias[l] = h->p.p + h->L[l].boff;
+            if (((uintptr_t)a.W[l] & 15) != 0)  // the kernel reads 16-byte weight fragments
+                throw MlpError("screening weights are not 16-byte aligned");
+        }
+        a.tm = h->dtm.p;
+        a.tol = tol;
+        a.period = dP.p;
+        a.amp = dA.p;
+        a.cls = dC.p;
+        a.flags = dF.p;
+        const unsigned int nblk = (unsigned int)((ncell + SROWS - 1) / SROWS);
+        hipLaunchKernelGGL(mlp_screen_fused, dim3(nblk), dim3(256), 0, h->stream, a);
+        check(hipGetLastError(), "mlp_screen_fused launch");
+    }
+    for (long long c0 = 0; !fused && c0 < ncell; c0 += chunk) {
         const int nb = (int)std::min<long long>(chunk, ncell - c0);
         hipLaunchKernelGGL(mlp_grid_inputs, dim3((nb + 255) / 256), dim3(256), 0, h->stream, c0, nb, nx, ny, h->X.p);
         check(hipGetLastError(), "mlp_grid_inputs launch");

@@ -125,6 +125,34 @@ def test_screen_flags_class_boundary(gpu_available):
     assert set(np.unique(F)) <= {0, 1, 2, 3, 4, 5, 6, 7}
 
 
+@pytest.mark.parametrize("shape", [(517, 389), (600, 500), (1, 1), (3, 129)])
+def test_screen_fused_matches_layer_by_layer(gpu_available, shape):
+    """The fused screening kernel (whole network per 128-cell block, the
+    activations in LDS) against the layer-by-layer products over chunks of
+    2^18 cells (CLODE_MLP_FUSED_SCREEN=0): same operand layout, K order and
+    epilogues, so period, amplitude, class and flags agree bit for bit.
+    Grids that are not multiples of the block, one that crosses the chunk
+    boundary (300,000 cells), a single cell and a one-row grid."""
+    m = net(11)
+    rng = np.random.default_rng(5)
+    U = rng.random((8192, 2))
+    T = np.stack([np.sin(4 * U[:, 0]), U[:, 1] ** 2], axis=1)
+    cls = (U[:, 0] + U[:, 1] > 1.0).astype(int) * 2
+    m.set_target_scaling([1.0, 0.5], [0.5, 2.0])
+    m.train(U, T, cls, epochs=4, batch=1024, lr=2e-3, seed=2)
+    nx, ny = shape
+    fused = m.screen_grid(nx, ny, tol=0.05, dilate=1)
+    os.environ["CLODE_MLP_FUSED_SCREEN"] = "0"
+    try:
+        layered = m.screen_grid(nx, ny, tol=0.05, dilate=1)
+    finally:
+        del os.environ["CLODE_MLP_FUSED_SCREEN"]
+    for a, b in zip(fused, layered):
+        assert a.shape == (nx, ny)
+        assert np.array_equal(a, b)
+    assert np.all(np.isfinite(fused[0])) and np.all(np.isfinite(fused[1]))
+
+
 def lactotroph_solver(n_axis_check=None):
     import clode_amd as clode
     from clode_amd.surrogate import integrator_solver
