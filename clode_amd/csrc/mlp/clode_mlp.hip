@@ -473,20 +473,31 @@ __global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float 
 }
 
 // minibatch gather: row r of the batch is sample perm[r] of the training
-// set; its inputs are encoded (with the transposed copy) in the same pass
+// set; its inputs are encoded (with the transposed copy) in the same pass.
+// One thread per (row, frequency), row fastest, so that a minibatch of a
+// few thousand rows spreads its sin/cos over 16x more lanes and the
+// transposed rows are written coalesced.
 __global__ void __launch_bounds__(256) mlp_gather(const unsigned int *perm, int nb, const float *Xa, const float *Ta,
                                                   const int *Ca, float *T, int *C, unsigned long long *ids,
                                                   const float *Bff, bf16_t *A0, bf16_t *A0t, int ldt)
 {
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nb)
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= nb * NFF)
         return;
+    const int r = idx % nb, f = idx / nb;
     const unsigned int j = perm[r];
-    T[2 * r] = Ta[2 * j];
-    T[2 * r + 1] = Ta[2 * j + 1];
-    C[r] = Ca[j];
-    ids[r] = j;
-    encode_row(Xa[2 * j], Xa[2 * j + 1], r, Bff, A0, A0t, ldt);
+    if (f == 0) {
+        T[2 * r] = Ta[2 * j];
+        T[2 * r + 1] = Ta[2 * j + 1];
+        C[r] = Ca[j];
+        ids[r] = j;
+    }
+    bf16_t sv, cv;
+    encode_pair(Xa[2 * j], Xa[2 * j + 1], f, Bff, sv, cv);
+    A0[(long long)r * D0 + 2 * f] = sv;
+    A0[(long long)r * D0 + 2 * f + 1] = cv;
+    A0t[(long long)(2 * f) * ldt + r] = sv;
+    A0t[(long long)(2 * f + 1) * ldt + r] = cv;
 }
 
 // ---------------------------------------------------------------- screening
@@ -495,8 +506,9 @@ __global__ void __launch_bounds__(256) mlp_gather(const unsigned int *perm, int 
 // class).  flags bit 0: the heads disagree (relative spread above tol, or a
 // head's class differs from the ensemble class).
 // z: the 72 head outputs of one cell
-__device__ __attribute__((always_inline)) inline void predict_row(const float *z, const float *tmean, const float *tstd, float tol, float &period,
-                                   float &amp, int &cls, uint8_t &flags)
+__device__ __attribute__((always_inline)) inline void predict_row(const float *z, const float *tmean,
+                                                                  const float *tstd, float tol, float &period,
+                                                                  float &amp, int &cls, uint8_t &flags)
 {
     // every loop has a constant trip count and is unrolled, so the arrays
     // live in registers (the first-occurrence argmax keeps its best value
@@ -1271,7 +1283,7 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
         for (size_t s = 0; s < steps_per_epoch; ++s, ++step) {
             const size_t b0 = s * (size_t)batch;
             const int nb = (int)std::min<size_t>((size_t)batch, n - b0);
-            hipLaunchKernelGGL(mlp_gather, dim3((nb + 255) / 256), dim3(256), 0, h->stream, dperm.p + b0, nb,
+            hipLaunchKernelGGL(mlp_gather, dim3((nb * NFF + 255) / 256), dim3(256), 0, h->stream, dperm.p + b0, nb,
                                Xall.p, Tall.p, Call.p, h->T.p, h->C.p, h->ids.p, h->dBff.p, h->A0.p, h->A0t.p,
                                ldT(nb));
             check(hipGetLastError(), "mlp_gather launch");
