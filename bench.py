@@ -500,7 +500,7 @@ def main():
                           "screen_tflops": 4194304 * 2 * 238720 / (net.last_ms() / 1e3) / 1e12}))
         return
     if args.gpus != 1:
-        print("bench.py: this build measures one GPU; multi-GPU runs are not part of it "
+        print("bench.py: measures one GPU; multi-GPU runs are not supported "
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
