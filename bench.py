@@ -156,7 +156,8 @@ def cpu_baseline(n_members=1024):
     wall = time.perf_counter() - t0
     return {"value": steps / wall, "unit": "member-steps/s", "cores": threads, "kind": "port",
             "sample": f"{n_members} of 65536 members (every {65536 // n_members}th), same transient+features "
-                      f"sequence, FP64, {steps} member-steps in {wall:.1f} s; CPU: {_cpu_model()}"}
+                      f"sequence, FP64, {steps} member-steps in {wall:.1f} s on {threads} threads "
+                      f"({wall * threads:.0f} core-seconds); CPU: {_cpu_model()}"}
 
 
 def _cpu_model():
