@@ -27,6 +27,7 @@ Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--no-extr
 from __future__ import annotations
 
 import argparse
+import inspect
 import json
 import os
 import sys
@@ -423,8 +424,8 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
 def surrogate_c5(precision, n=2048):
     """2048 x 2048 lactotroph feature map (gK in [0, 10] x gBK in [0, 2],
     dopri5 rtol 1e-6, transient 5e3 then thresh2 over 2e4): the plain map
-    against the surrogate-screened map (2^17 integrated training samples,
-    BF16-MFMA MLP, screening, re-solve of the flagged cells)."""
+    against the surrogate-screened map (2^18 integrated training samples, 10
+    epochs, BF16-MFMA MLP, screening, re-solve of the flagged cells)."""
     import clode_amd as clode
     from clode_amd.surrogate import compare_maps, integrator_solver, screened_map
 
@@ -445,7 +446,10 @@ def surrogate_c5(precision, n=2048):
     plain = {k: v.reshape(n, n) for k, v in plain.items()}
     m, _ = screened_map(solve, n, n, seed=1)
     bad = compare_maps({"period": m.period, "amp": m.amp, "cls": m.cls}, plain, tol=0.05)
-    return {"grid": f"{n}x{n}", "plain_map_s": plain_s, "screened_total_s": m.timings["total_s"],
+    defaults = inspect.signature(screened_map).parameters
+    return {"grid": f"{n}x{n}", "training_samples": defaults["n_train"].default,
+            "epochs": defaults["epochs"].default,
+            "plain_map_s": plain_s, "screened_total_s": m.timings["total_s"],
             "speedup_total": plain_s / m.timings["total_s"],
             "speedup_screen_and_resolve": plain_s / (m.timings["screen_s"] + m.timings["resolve_s"]),
             "timings": m.timings, "cells_integrated_frac": float(m.solved.mean()),
