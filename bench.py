@@ -489,7 +489,16 @@ def main():
                          "2048 x 2048 screening pass), for profiling")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="run only a 256 MiB device copy kernel (FETCH_SIZE/WRITE_SIZE calibration)")
+    ap.add_argument("--calibrate-fma", action="store_true",
+                    help="run only the FP32/FP64 FMA throughput probe (achievable vector-ALU peaks)")
     args = ap.parse_args()
+    if args.calibrate_fma:
+        import subprocess
+
+        probe = os.path.join(REPO, "clode_amd", "_build", "valu_rate")
+        if not os.path.exists(probe):
+            sys.exit("bench.py: the FMA probe is not built (run __graft_entry__.build())")
+        sys.exit(subprocess.run([probe]).returncode)
     if args.calibrate_copy:
         import torch
 
