@@ -424,8 +424,9 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
 def surrogate_c5(precision, n=2048):
     """2048 x 2048 lactotroph feature map (gK in [0, 10] x gBK in [0, 2],
     dopri5 rtol 1e-6, transient 5e3 then thresh2 over 2e4): the plain map
-    against the surrogate-screened map (2^18 integrated training samples, 10
-    epochs, BF16-MFMA MLP, screening, re-solve of the flagged cells)."""
+    against the surrogate-screened map (2^18 integrated training samples,
+    BF16-MFMA MLP trained with screened_map's defaults, screening, re-solve of
+    the flagged cells)."""
     import clode_amd as clode
     from clode_amd.surrogate import compare_maps, integrator_solver, screened_map
 
@@ -448,7 +449,7 @@ def surrogate_c5(precision, n=2048):
     bad = compare_maps({"period": m.period, "amp": m.amp, "cls": m.cls}, plain, tol=0.05)
     defaults = inspect.signature(screened_map).parameters
     return {"grid": f"{n}x{n}", "training_samples": defaults["n_train"].default,
-            "epochs": defaults["epochs"].default,
+            "epochs": defaults["epochs"].default, "dilation": defaults["dilate"].default,
             "plain_map_s": plain_s, "screened_total_s": m.timings["total_s"],
             "speedup_total": plain_s / m.timings["total_s"],
             "speedup_screen_and_resolve": plain_s / (m.timings["screen_s"] + m.timings["resolve_s"]),
