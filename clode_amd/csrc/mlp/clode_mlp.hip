@@ -1023,6 +1023,15 @@ float loss_grad(clode_mlp *h, int n)
     return l;
 }
 
+// a caller's pointer that must not be null
+template <class T>
+T *need(T *p, const char *what)
+{
+    if (!p)
+        throw std::invalid_argument(std::string("null ") + what);
+    return p;
+}
+
 #define MLP_TRY try {
 #define MLP_CATCH                        \
     }                                    \
@@ -1148,7 +1157,7 @@ int clode_mlp_destroy(clode_mlp *h)
 
 int clode_mlp_n_params(clode_mlp *h, size_t *n)
 {
-    CHECK_H(h) MLP_TRY *n = h->nparam;
+    CHECK_H(h) MLP_TRY *need(n, "n") = h->nparam;
     MLP_CATCH
 }
 
@@ -1157,6 +1166,7 @@ int clode_mlp_get_params(clode_mlp *h, float *out, size_t n)
     CHECK_H(h) MLP_TRY
     if (n != h->nparam)
         throw std::invalid_argument("parameter vector has the wrong length");
+    need(out, "output buffer");
     check(hipSetDevice(h->device), "hipSetDevice");
     download(h, out, h->p.p, n * 4);
     MLP_CATCH
@@ -1167,6 +1177,7 @@ int clode_mlp_set_params(clode_mlp *h, const float *in, size_t n)
     CHECK_H(h) MLP_TRY
     if (n != h->nparam)
         throw std::invalid_argument("parameter vector has the wrong length");
+    need(in, "parameter vector");
     check(hipSetDevice(h->device), "hipSetDevice");
     upload(h, h->p.p, in, n * 4);
     refresh_bf16(h);
@@ -1178,13 +1189,15 @@ int clode_mlp_get_fourier(clode_mlp *h, float *out, size_t n)
     CHECK_H(h) MLP_TRY
     if (n != 2 * NFF)
         throw std::invalid_argument("Fourier matrix has 32 entries");
-    std::copy(h->Bff.begin(), h->Bff.end(), out);
+    std::copy(h->Bff.begin(), h->Bff.end(), need(out, "output buffer"));
     MLP_CATCH
 }
 
 int clode_mlp_set_target_scaling(clode_mlp *h, const float *mean2, const float *std2)
 {
     CHECK_H(h) MLP_TRY
+    need(mean2, "mean2");
+    need(std2, "std2");
     for (int k = 0; k < 2; ++k) {
         if (!(std2[k] > 0))
             throw std::invalid_argument("target scales must be positive");
@@ -1199,6 +1212,8 @@ int clode_mlp_forward(clode_mlp *h, const float *X, size_t n, float *out, unsign
     CHECK_H(h) MLP_TRY
     if (n == 0 || n > (1u << 18))
         throw std::invalid_argument("forward takes 1 to 262144 rows");
+    need(X, "input rows");
+    need(out, "output buffer");
     check(hipSetDevice(h->device), "hipSetDevice");
     h->ensureRows(n);
     upload(h, h->X.p, X, n * 8);
@@ -1219,6 +1234,10 @@ int clode_mlp_loss_grad(clode_mlp *h, const float *X, const float *T, const int 
     CHECK_H(h) MLP_TRY
     if (n == 0 || n > (1u << 16))
         throw std::invalid_argument("loss_grad takes 1 to 65536 rows");
+    need(X, "input rows");
+    need(T, "targets");
+    need(cls, "class labels");
+    need(loss, "loss output");
     for (size_t i = 0; i < n; ++i)
         if (cls[i] < 0 || cls[i] >= NCLS)
             throw std::invalid_argument("class labels must be in 0..15");
@@ -1243,6 +1262,9 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
     CHECK_H(h) MLP_TRY
     if (n < 1 || n > (1u << 24) || epochs < 1 || batch < 64 || batch > 65536)
         throw std::invalid_argument("train: 1..16777216 samples, epochs >= 1, batch 64..65536");
+    need(X, "input rows");
+    need(T, "targets");
+    need(cls, "class labels");
     for (size_t i = 0; i < n; ++i)
         if (cls[i] < 0 || cls[i] >= NCLS)
             throw std::invalid_argument("class labels must be in 0..15");
@@ -1324,6 +1346,10 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
         throw std::invalid_argument("grid must have 1 to 2^26 cells");
     if (dilate < 0 || dilate > 64)
         throw std::invalid_argument("dilation radius must be 0..64");
+    need(period, "period output");
+    need(amp, "amplitude output");
+    need(cls, "class output");
+    need(flags, "flags output");
     check(hipSetDevice(h->device), "hipSetDevice");
     const long long ncell = (long long)nx * ny;
     // the fused kernel by default; CLODE_MLP_FUSED_SCREEN=0 runs the network
@@ -1398,7 +1424,7 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
 
 int clode_mlp_last_ms(clode_mlp *h, double *ms)
 {
-    CHECK_H(h) MLP_TRY *ms = h->lastMs;
+    CHECK_H(h) MLP_TRY *need(ms, "ms") = h->lastMs;
     MLP_CATCH
 }
 

@@ -58,6 +58,23 @@ static thread_local std::string g_last_error;
         return CLODE_ERR_INVALID;                             \
     }
 
+// a caller's pointer that must not be null
+template <class T>
+static T *need(T *p, const char *what)
+{
+    if (!p)
+        throw InvalidArgument(std::string("null ") + what);
+    return p;
+}
+
+// a caller's input array of n elements (null only when n == 0)
+static std::vector<double> in_vec(const double *p, size_t n, const char *what)
+{
+    if (!p && n > 0)
+        throw InvalidArgument(std::string("null ") + what);
+    return p ? std::vector<double>(p, p + n) : std::vector<double>();
+}
+
 static std::vector<std::string> split_names(const char *s)
 {
     std::vector<std::string> out;
@@ -157,7 +174,7 @@ const char *clode_version(void) { return "clode_amd 0.1.0 (gfx950, hipRTC)"; }
 int clode_device_count(int *count)
 {
     CLODE_TRY
-    *count = device_count();
+    *need(count, "count") = device_count();
     CLODE_CATCH
 }
 
@@ -242,23 +259,22 @@ int clode_compile(clode_sim *sim)
 int clode_set_problem_data(clode_sim *sim, const double *x0, size_t nx0, const double *pars, size_t npars)
 {
     CHECK_SIM(sim) CLODE_TRY
-    std::vector<double> a(x0, x0 + nx0), b(pars, pars + npars);
-    sim->s->setProblemData(a, b);
+    sim->s->setProblemData(in_vec(x0, nx0, "x0"), in_vec(pars, npars, "pars"));
     CLODE_CATCH
 }
 int clode_set_tspan(clode_sim *sim, const double *tspan, size_t n)
 {
-    CHECK_SIM(sim) CLODE_TRY sim->s->setTspan(std::vector<double>(tspan, tspan + n));
+    CHECK_SIM(sim) CLODE_TRY sim->s->setTspan(in_vec(tspan, n, "tspan"));
     CLODE_CATCH
 }
 int clode_set_x0(clode_sim *sim, const double *x0, size_t n)
 {
-    CHECK_SIM(sim) CLODE_TRY sim->s->setX0(std::vector<double>(x0, x0 + n));
+    CHECK_SIM(sim) CLODE_TRY sim->s->setX0(in_vec(x0, n, "x0"));
     CLODE_CATCH
 }
 int clode_set_pars(clode_sim *sim, const double *pars, size_t n)
 {
-    CHECK_SIM(sim) CLODE_TRY sim->s->setPars(std::vector<double>(pars, pars + n));
+    CHECK_SIM(sim) CLODE_TRY sim->s->setPars(in_vec(pars, n, "pars"));
     CLODE_CATCH
 }
 int clode_set_solver_params(clode_sim *sim, const clode_solver_params *sp)
@@ -306,12 +322,12 @@ int clode_shift_x0(clode_sim *sim)
 
 int clode_get_n_pts(clode_sim *sim, int *n)
 {
-    CHECK_SIM(sim) CLODE_TRY *n = sim->s->nPts;
+    CHECK_SIM(sim) CLODE_TRY *need(n, "n") = sim->s->nPts;
     CLODE_CATCH
 }
 int clode_get_tspan(clode_sim *sim, double *t)
 {
-    CHECK_SIM(sim) CLODE_TRY t[0] = sim->s->tspan[0];
+    CHECK_SIM(sim) CLODE_TRY need(t, "tspan output")[0] = sim->s->tspan[0];
     t[1] = sim->s->tspan[1];
     CLODE_CATCH
 }
@@ -319,6 +335,7 @@ int clode_get_solver_params(clode_sim *sim, clode_solver_params *sp)
 {
     CHECK_SIM(sim) CLODE_TRY
     const SolverParams &p = sim->s->sp;
+    need(sp, "solver params output");
     sp->dt = p.dt;
     sp->dtmax = p.dtmax;
     sp->abstol = p.abstol;
@@ -370,15 +387,15 @@ int clode_get_program_string(clode_sim *sim, char *buf, size_t cap, size_t *need
 int clode_get_problem_info(clode_sim *sim, int *nvar, int *npar, int *naux, int *nnoise)
 {
     CHECK_SIM(sim) CLODE_TRY
-    *nvar = sim->s->prob.nVar;
-    *npar = sim->s->prob.nPar;
-    *naux = sim->s->prob.nAux;
-    *nnoise = sim->s->prob.nWiener;
+    *need(nvar, "nvar") = sim->s->prob.nVar;
+    *need(npar, "npar") = sim->s->prob.nPar;
+    *need(naux, "naux") = sim->s->prob.nAux;
+    *need(nnoise, "nnoise") = sim->s->prob.nWiener;
     CLODE_CATCH
 }
 int clode_get_precision(clode_sim *sim, int *precision)
 {
-    CHECK_SIM(sim) CLODE_TRY *precision = sim->s->prec;
+    CHECK_SIM(sim) CLODE_TRY *need(precision, "precision") = sim->s->prec;
     CLODE_CATCH
 }
 int clode_get_stepper(clode_sim *sim, char *buf, size_t cap, size_t *needed)
@@ -423,7 +440,7 @@ int clode_initialize_observer(clode_sim *sim)
 }
 int clode_is_observer_initialized(clode_sim *sim, int *initialized)
 {
-    CHECK_SIM(sim) CLODE_TRY *initialized = sim->s->observerInitialized ? 1 : 0;
+    CHECK_SIM(sim) CLODE_TRY *need(initialized, "initialized") = sim->s->observerInitialized ? 1 : 0;
     CLODE_CATCH
 }
 int clode_features(clode_sim *sim, int reinitialize)
@@ -441,6 +458,7 @@ int clode_get_observer_params(clode_sim *sim, clode_observer_params *o)
 {
     CHECK_SIM(sim) CLODE_TRY
     const ObserverParams &p = sim->s->op;
+    need(o, "observer params output");
     o->eVarIx = p.eVarIx;
     o->fVarIx = p.fVarIx;
     o->maxEventCount = p.maxEventCount;
@@ -462,7 +480,7 @@ int clode_get_observer_name(clode_sim *sim, char *buf, size_t cap, size_t *neede
 }
 int clode_get_n_features(clode_sim *sim, int *n)
 {
-    CHECK_SIM(sim) CLODE_TRY *n = sim->s->nFeatures;
+    CHECK_SIM(sim) CLODE_TRY *need(n, "n") = sim->s->nFeatures;
     CLODE_CATCH
 }
 int clode_get_f(clode_sim *sim, double *out, size_t n)
@@ -496,9 +514,9 @@ int clode_trajectory(clode_sim *sim)
 int clode_get_trajectory_sizes(clode_sim *sim, size_t *nt, size_t *nx, size_t *naux)
 {
     CHECK_SIM(sim) CLODE_TRY
-    *nt = sim->s->telements;
-    *nx = sim->s->xelements;
-    *naux = sim->s->auxelements;
+    *need(nt, "nt") = sim->s->telements;
+    *need(nx, "nx") = sim->s->xelements;
+    *need(naux, "naux") = sim->s->auxelements;
     CLODE_CATCH
 }
 int clode_get_t(clode_sim *sim, double *out, size_t n)
@@ -527,6 +545,8 @@ int clode_get_n_stored(clode_sim *sim, int *out, size_t n)
     std::vector<int> v = sim->s->getNstored();
     if (n < v.size())
         throw InvalidArgument("output buffer too small");
+    if (!v.empty())
+        need(out, "output buffer");
     std::copy(v.begin(), v.end(), out);
     CLODE_CATCH
 }
@@ -538,7 +558,7 @@ int clode_set_log_level(int level)
 }
 int clode_get_log_level(int *level)
 {
-    CLODE_TRY *level = get_log_level();
+    CLODE_TRY *need(level, "level") = get_log_level();
     CLODE_CATCH
 }
 
@@ -608,8 +628,8 @@ int clode_batch_set_launch(clode_batch *batch, int block_size, int slice)
 int clode_batch_get_launch(clode_batch *batch, int *block_size, int *slice)
 {
     CHECK_BATCH(batch) CLODE_TRY
-    *block_size = batch->b->block();
-    *slice = batch->b->slice();
+    *need(block_size, "block_size") = batch->b->block();
+    *need(slice, "slice") = batch->b->slice();
     CLODE_CATCH
 }
 int clode_batch_transient(clode_batch *batch)
@@ -653,7 +673,7 @@ int clode_batch_autotune(clode_batch *batch, int stride, unsigned int max_steps,
 }
 int clode_batch_last_ms(clode_batch *batch, double *ms)
 {
-    CHECK_BATCH(batch) CLODE_TRY *ms = batch->b->lastMs;
+    CHECK_BATCH(batch) CLODE_TRY *need(ms, "ms") = batch->b->lastMs;
     CLODE_CATCH
 }
 
@@ -680,7 +700,7 @@ int clode_set_dt_refresh(clode_sim *sim, int enable)
 }
 int clode_last_kernel_ms(clode_sim *sim, double *ms)
 {
-    CHECK_SIM(sim) CLODE_TRY *ms = sim->s->lastKernelMs;
+    CHECK_SIM(sim) CLODE_TRY *need(ms, "ms") = sim->s->lastKernelMs;
     CLODE_CATCH
 }
 int clode_set_step_counter(clode_sim *sim, int enable)
@@ -699,6 +719,8 @@ int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n)
     std::vector<unsigned long long> v = sim->s->getStepCounts();
     if (n < v.size())
         throw InvalidArgument("output buffer too small");
+    if (!v.empty())
+        need(out, "output buffer");
     std::copy(v.begin(), v.end(), out);
     CLODE_CATCH
 }

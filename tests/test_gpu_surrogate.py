@@ -61,6 +61,27 @@ def test_forward_large_tiles(gpu_available):
     assert np.median(err) < 1e-4
 
 
+def test_null_pointers_are_invalid_arguments(gpu_available):
+    """Every entry point refuses a null array or output pointer with
+    CLODE_MLP_ERR_INVALID before any device work."""
+    from clode_amd.surrogate import _lib
+
+    lib, m = _lib(), net(0)
+    h = m._h
+    for fn, args in [("clode_mlp_n_params", [None]), ("clode_mlp_get_params", [None, m.n_params]),
+                     ("clode_mlp_set_params", [None, m.n_params]), ("clode_mlp_get_fourier", [None, 32]),
+                     ("clode_mlp_set_target_scaling", [None, None]),
+                     ("clode_mlp_forward", [None, 10, None, None]),
+                     ("clode_mlp_loss_grad", [None, None, None, 10, None, None]),
+                     ("clode_mlp_train", [None, None, None, 10, 1, 64, 1e-3, 0, None]),
+                     ("clode_mlp_screen_grid", [4, 4, 0.1, 0, None, None, None, None]),
+                     ("clode_mlp_last_ms", [None])]:
+        assert getattr(lib, fn)(h, *args) == 1, fn
+        assert b"null" in lib.clode_mlp_last_error(), fn
+    # the handle still works afterwards
+    assert m.forward(np.zeros((3, 2), np.float32)).shape[0] == 3
+
+
 def test_gradient_matches_oracle(gpu_available):
     m = net(5)
     rng = np.random.default_rng(1)

@@ -54,6 +54,39 @@ def test_c_abi_without_device():
     assert b"null" in lib.clode_last_error()
 
 
+class _ProblemInfo(ctypes.Structure):
+    _fields_ = [("src_file", ctypes.c_char_p), ("num_var", ctypes.c_int), ("num_par", ctypes.c_int),
+                ("num_aux", ctypes.c_int), ("num_noise", ctypes.c_int), ("var_names", ctypes.c_char_p),
+                ("par_names", ctypes.c_char_p), ("aux_names", ctypes.c_char_p)]
+
+
+def test_c_abi_null_pointers_are_invalid_arguments():
+    """Null output pointers and null input arrays with a nonzero size are
+    refused with CLODE_ERR_INVALID and a message, never dereferenced."""
+    lib = ctypes.CDLL(LIB)
+    lib.clode_last_error.restype = ctypes.c_char_p
+    assert lib.clode_device_count(None) == 1
+    assert lib.clode_get_log_level(None) == 1
+    pi = _ProblemInfo(os.path.join(MODELS, "van_der_pol.cl").encode(), 2, 1, 0, 0, b"x\ny", b"mu", None)
+    sim = ctypes.c_void_p()
+    assert lib.clode_create(ctypes.byref(sim), 0, ctypes.byref(pi), b"rk4", None, None, 2, 0, None) == 0
+    try:
+        for fn, args in [("clode_get_n_pts", [None]), ("clode_get_tspan", [None]),
+                         ("clode_get_solver_params", [None]), ("clode_get_precision", [None]),
+                         ("clode_get_problem_info", [None, None, None, None]),
+                         ("clode_is_observer_initialized", [None]), ("clode_last_kernel_ms", [None]),
+                         ("clode_set_x0", [None, ctypes.c_size_t(4)]),
+                         ("clode_set_pars", [None, ctypes.c_size_t(2)]),
+                         ("clode_set_tspan", [None, ctypes.c_size_t(2)]),
+                         ("clode_set_problem_data", [None, ctypes.c_size_t(2), None, ctypes.c_size_t(1)])]:
+            assert getattr(lib, fn)(sim, *args) == 1, fn
+            assert b"null" in lib.clode_last_error(), fn
+        n = ctypes.c_int(-1)
+        assert lib.clode_get_n_pts(sim, ctypes.byref(n)) == 0 and n.value == 0
+    finally:
+        assert lib.clode_destroy(sim) == 0
+
+
 def test_pybind_defaults_match_reference():
     from clode_amd.cpp import clode_cpp_wrapper as w
 
