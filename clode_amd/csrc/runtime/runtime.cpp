@@ -455,6 +455,10 @@ std::shared_ptr<Program> get_program_source(const std::string &source, const std
         keytext += "\n" + o;
     for (const auto &h : kDeviceSources)
         keytext += h.text;
+    // a different compiler must not reuse code objects cached by another one
+    int rtc_major = 0, rtc_minor = 0;
+    if (hiprtcVersion(&rtc_major, &rtc_minor) == HIPRTC_SUCCESS)
+        keytext += "\nhiprtc " + std::to_string(rtc_major) + "." + std::to_string(rtc_minor);
     const uint64_t key = fnv1a(keytext);
 
     std::lock_guard<std::mutex> lk(g_prog_mu);
