@@ -371,14 +371,16 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
     return o;
 }
 
-static uint64_t fnv1a(const std::string &s, uint64_t h = 1469598103934665603ull)
+static uint64_t fnv1a(const char *p, size_t n, uint64_t h = 1469598103934665603ull)
 {
-    for (unsigned char c : s) {
-        h ^= c;
+    for (size_t i = 0; i < n; ++i) {
+        h ^= (unsigned char)p[i];
         h *= 1099511628211ull;
     }
     return h;
 }
+
+static uint64_t fnv1a(const std::string &s) { return fnv1a(s.data(), s.size()); }
 
 static std::mutex g_prog_mu;
 // never destroyed: modules stay loaded until the process exits, so no HIP call
@@ -395,21 +397,30 @@ static std::string cache_path(const std::string &dir, uint64_t key)
     return d + name;
 }
 
+// cache file: "CLODECO2", the program key, the code size and an FNV-1a
+// checksum of the code (each 8 bytes), then the code object.  A file whose
+// header or checksum does not match is ignored and rebuilt.
 static bool read_cache(const std::string &path, uint64_t key, std::vector<char> &code)
 {
     std::ifstream f(path, std::ios::binary);
     if (!f)
         return false;
     char magic[8];
-    uint64_t k = 0, n = 0;
+    uint64_t k = 0, n = 0, sum = 0;
     f.read(magic, 8);
     f.read((char *)&k, 8);
     f.read((char *)&n, 8);
-    if (!f || std::memcmp(magic, "CLODECO1", 8) != 0 || k != key || n == 0 || n > (1ull << 30))
+    f.read((char *)&sum, 8);
+    if (!f || std::memcmp(magic, "CLODECO2", 8) != 0 || k != key || n == 0 || n > (1ull << 30))
         return false;
-    code.resize(n);
-    f.read(code.data(), (std::streamsize)n);
-    return (bool)f;
+    std::vector<char> body(n);
+    f.read(body.data(), (std::streamsize)n);
+    if (!f || fnv1a(body.data(), n) != sum) {
+        log_msg(LOG_WARN, "ignoring damaged cached code object " + path);
+        return false;
+    }
+    code.swap(body);
+    return true;
 }
 
 static void write_cache(const std::string &dir, const std::string &path, uint64_t key, const std::vector<char> &code)
@@ -422,13 +433,17 @@ static void write_cache(const std::string &dir, const std::string &path, uint64_
         std::ofstream f(tmp, std::ios::binary);
         if (!f)
             return;
-        uint64_t n = code.size();
-        f.write("CLODECO1", 8);
+        const uint64_t n = code.size(), sum = fnv1a(code.data(), code.size());
+        f.write("CLODECO2", 8);
         f.write((const char *)&key, 8);
         f.write((const char *)&n, 8);
+        f.write((const char *)&sum, 8);
         f.write(code.data(), (std::streamsize)n);
-        if (!f)
-            return;
+    }
+    std::ifstream check(tmp, std::ios::binary | std::ios::ate);
+    if (!check || (uint64_t)check.tellg() != 32 + (uint64_t)code.size()) {
+        std::remove(tmp.c_str());
+        return;
     }
     std::rename(tmp.c_str(), path.c_str());
 }

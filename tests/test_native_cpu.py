@@ -180,6 +180,47 @@ def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
     assert f"#define CLODE_PREC {dict(single=1, double=2, fp32c=3)[prec]}" in prog
 
 
+_COMPILE_INTO = """
+import sys
+from clode_amd.cpp import clode_cpp_wrapper as w
+pi = w.ProblemInfo(sys.argv[1], ["x", "y"], ["mu"], [], 0)
+w.SimulatorBase(pi, "rk4", "double", w.OpenCLResource(), sys.argv[2]).compile()
+"""
+
+
+def test_code_object_cache_checksum(tmp_path):
+    """The on-disk code-object cache: a build writes one file (magic, key,
+    size, checksum, code); a fresh process reuses it untouched; a file with a
+    damaged body is ignored, rebuilt and rewritten."""
+    src = os.path.join(MODELS, "van_der_pol.cl")
+
+    def compile_in_fresh_process():
+        subprocess.run(["python3", "-c", _COMPILE_INTO, src, str(tmp_path)], check=True, cwd=REPO)
+
+    compile_in_fresh_process()
+    files = sorted(tmp_path.glob("clode_*.co"))
+    assert len(files) == 1
+    path = files[0]
+    data = path.read_bytes()
+    assert data[:8] == b"CLODECO2"
+    n = int.from_bytes(data[16:24], "little")
+    assert len(data) == 32 + n
+    h = 1469598103934665603
+    for c in data[32:]:
+        h = ((h ^ c) * 1099511628211) & (2**64 - 1)
+    assert int.from_bytes(data[24:32], "little") == h
+    stamp = path.stat().st_mtime_ns
+    compile_in_fresh_process()
+    assert path.stat().st_mtime_ns == stamp  # intact: reused, not rewritten
+    damaged = bytearray(data)
+    damaged[32 + n // 2] ^= 0xFF
+    path.write_bytes(bytes(damaged))
+    compile_in_fresh_process()
+    rebuilt = path.read_bytes()
+    assert rebuilt[:32] == data[:32] and rebuilt == data
+    assert not list(tmp_path.glob("*.tmp*"))
+
+
 def test_unknown_names_and_bad_input():
     from clode_amd.cpp import clode_cpp_wrapper as w
 
