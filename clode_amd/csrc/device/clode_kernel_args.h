@@ -50,7 +50,7 @@ struct ClodeArgs {
     void *tr_aux;        // [maxStore][nAux][nPts]
     int *nStored;        // [nPts]
     void *tspan_m;               // optional per-member [2][nPts] time spans (double)
-    unsigned long long *steps;   // optional per-member step counter (accumulated)
+    unsigned long long *steps;   // optional [2][nPts]: accepted steps, then step trials (accumulated)
     // work queue (persistent launches): position q -> member q_order[q*q_stride]
     unsigned int *work_counter;  // next queue position to claim; zeroed before each launch
     const unsigned int *q_order; // optional queue order (member indices)

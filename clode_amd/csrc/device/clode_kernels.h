@@ -176,11 +176,14 @@ CLODE_DEV void store_member(Member &m, const ClodeArgs &a, unsigned int i)
     ((dtstore *)a.tf)[i] = m.ti;
 }
 
-// optional per-member accumulation of member-steps (benchmark accounting)
-CLODE_DEV void count_steps(const ClodeArgs &a, unsigned int i, unsigned int step)
+// optional per-member accumulation of member-steps and of step trials (one
+// per stepper call: a step plus its rejected trials), for benchmark accounting
+CLODE_DEV void count_steps(const ClodeArgs &a, unsigned int i, unsigned int step, unsigned int trials)
 {
-    if (a.steps)
+    if (a.steps) {
         a.steps[i] += step;
+        a.steps[(size_t)a.nPts + i] += trials;
+    }
 }
 
 // the member's time span: the launch's, or its own when tspan_m is given
@@ -217,7 +220,7 @@ CLODE_DEV void begin_member(Member &m, const ClodeArgs &a, unsigned int i, timet
 struct TransientLane {
     Member m;
     timetype t1;
-    unsigned int i, step;
+    unsigned int i, step, trials;
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
         i = idx;
@@ -225,17 +228,19 @@ struct TransientLane {
         member_tspan(a, i, t0, t1);
         begin_member(m, a, i, t0);
         step = 0;
+        trials = 0;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const { return m.ti <= t1 && step < sp.max_steps; }
     CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
     {
+        ++trials;
         if (CLODE_STEP(m))
             ++step;
     }
     CLODE_DEV void finish(const ClodeArgs &a)
     {
         store_member(m, a, i);
-        count_steps(a, i, step);
+        count_steps(a, i, step, trials);
     }
 };
 
@@ -249,7 +254,7 @@ struct InitObsLane {
     Observer ob;
     ObserverParamsR op;
     timetype t0, t1;
-    unsigned int i, step;
+    unsigned int i, step, trials;
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
         i = idx;
@@ -259,6 +264,7 @@ struct InitObsLane {
         ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
         ob.init(m.ti, m.xi, m.dxi, m.auxi, op, io);
         step = 0;
+        trials = 0;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const
     {
@@ -266,6 +272,7 @@ struct InitObsLane {
     }
     CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
     {
+        ++trials;
         if (!CLODE_STEP(m))
             return;  // trial rejected: retried on the next iteration
         ++step;
@@ -277,7 +284,7 @@ struct InitObsLane {
             m.ti = t0;
             load_state(m, a, i);
             getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
-            count_steps(a, i, step);
+            count_steps(a, i, step, trials);
         }
         ob.initEventDetector(m.ti, m.xi, m.dxi, m.auxi, op);
         ObsStore st{ObsIO{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0}};
@@ -290,7 +297,7 @@ struct FeaturesLane {
     Observer ob;
     ObserverParamsR op;
     timetype t0, t1;
-    unsigned int i, step;
+    unsigned int i, step, trials;
     bool terminal;
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
@@ -306,11 +313,13 @@ struct FeaturesLane {
         initial_noise(m);
         getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
         step = 0;
+        trials = 0;
         terminal = false;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const { return !terminal && m.ti <= t1 && step < sp.max_steps; }
     CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
     {
+        ++trials;
         if (!CLODE_STEP(m))
             return;  // trial rejected: retried on the next iteration
         const ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
@@ -328,7 +337,7 @@ struct FeaturesLane {
         ObsStore st{io};
         ob.fields(st);
         store_member(m, a, i);
-        count_steps(a, i, step);
+        count_steps(a, i, step, trials);
     }
 };
 #endif  // CLODE_OBSERVER
@@ -353,7 +362,7 @@ CLODE_DEV void trajectory_store(const ClodeArgs &a, unsigned int i, unsigned int
 struct TrajectoryLane {
     Member m;
     timetype t1;
-    unsigned int i, step, storeix;
+    unsigned int i, step, storeix, trials;
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
         i = idx;
@@ -364,6 +373,7 @@ struct TrajectoryLane {
         if (a.sp_max_store > 0)
             trajectory_store(a, i, storeix, m);
         step = 0;
+        trials = 0;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const
     {
@@ -377,8 +387,9 @@ struct TrajectoryLane {
         // buffers.  (Letting rejected lanes fall behind spreads a wave's
         // stores over many sample rows; measured 2.7x slower on 65,536
         // members x 1024 samples.)
-        while (!CLODE_STEP(m)) {
-        }
+        ++trials;
+        while (!CLODE_STEP(m))
+            ++trials;
         ++step;
         if (step % sp.nout == 0) {
             ++storeix;
@@ -390,7 +401,7 @@ struct TrajectoryLane {
     {
         a.nStored[i] = (int)storeix;
         store_member(m, a, i);
-        count_steps(a, i, step);
+        count_steps(a, i, step, trials);
     }
 };
 

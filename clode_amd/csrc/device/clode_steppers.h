@@ -42,11 +42,18 @@
 #define CLODE_STOCHASTIC 1
 #endif
 
-// Bound on step attempts per accepted step.  The reference's retry loop
-// (adaptive_explicit_step.clh:20-62) ends at hmin; if rounding of t + dt keeps
-// the attempted step just above hmin it would not end, so after this many
-// attempts the step is treated as an hmin failure.
-#define CLODE_MAX_ATTEMPTS 100
+// End of the adaptive retry loop.  The reference's loop
+// (adaptive_explicit_step.clh:20-62) ends when a rejected trial's step is at
+// most hmin.  After a rejection the next trial's (clamped) step is strictly
+// smaller until it is clamped at hmin, so a trial equal to the previous one
+// can only happen at hmin, when t + hmin rounds to a step longer than hmin
+// (t just below a power of two): that trial would repeat, and be rejected,
+// forever.  It is failed at hmin instead (stall detection).  Every sequence
+// that ends in the reference ends here after the same trials.  As a second
+// guard a step makes at most CLODE_MAX_ATTEMPTS trials; no sequence that
+// ends in the reference gets near it (from the largest double down to hmin
+// takes about 2300 trials, in float about 300).
+#define CLODE_MAX_ATTEMPTS 4096
 
 #define N_PAR1 (N_PAR > 0 ? N_PAR : 1)
 #define N_AUX1 (N_AUX > 0 ? N_AUX : 1)
@@ -77,7 +84,8 @@ typedef double ctlreal;
 typedef realtype ctlreal;
 #endif
 struct StepCtl {
-    ctlreal newDt;
+    ctlreal newDt;   // the next trial's step before clamping
+    ctlreal lastDt;  // the previous trial's clamped step (stall detection)
     int attempts;
     bool noFailedSteps;
     bool inStep;
@@ -85,6 +93,7 @@ struct StepCtl {
 CLODE_DEV void step_ctl_reset(StepCtl &c)
 {
     c.newDt = 0;
+    c.lastDt = 0;
     c.attempts = 0;
     c.noFailedSteps = true;
     c.inStep = false;
@@ -287,17 +296,18 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
         c.attempts = 0;
         c.newDt = dt;
     }
-    if (++c.attempts > CLODE_MAX_ATTEMPTS) {
-        dt = hmin;
+    realtype newDt = fmin(fmax(c.newDt, hmin), sp.dtmax);
+    if ((!c.noFailedSteps && newDt == c.lastDt) || ++c.attempts > CLODE_MAX_ATTEMPTS) {
+        dt = hmin;  // stalled at hmin: the reference would retry this trial forever
         c.inStep = false;
         return true;
     }
+    c.lastDt = newDt;
     tNew = ti;
     for (int j = 0; j < N_VAR; j++) {
         newxi[j] = xi[j];
         newk1[j] = k1[j];
     }
-    realtype newDt = fmin(fmax(c.newDt, hmin), sp.dtmax);
     newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, wi);
     for (int j = 0; j < N_VAR; j++)
         err[j] /= fmax(fmax(fabs(xi[j]), fabs(newxi[j])), threshold);
@@ -524,17 +534,18 @@ CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k
         c.attempts = 0;
         c.newDt = dt;
     }
-    if (++c.attempts > CLODE_MAX_ATTEMPTS) {
-        dt = hmin;
+    double newDt = fmin(fmax(c.newDt, hmin), dtmax);
+    if ((!c.noFailedSteps && newDt == c.lastDt) || ++c.attempts > CLODE_MAX_ATTEMPTS) {
+        dt = hmin;  // stalled at hmin (see CLODE_MAX_ATTEMPTS)
         c.inStep = false;
         return true;
     }
+    c.lastDt = newDt;
     tNew = ti;
     for (int j = 0; j < N_VAR; j++) {
         newxi[j] = xi[j];
         newk1[j] = k1[j];
     }
-    double newDt = fmin(fmax(c.newDt, hmin), dtmax);
     newDt = do_step(tNew, newxi, newk1, pars, newDt, aux, err, dx, wi);
     normErr = 0.0;
     for (int j = 0; j < N_VAR; j++) {

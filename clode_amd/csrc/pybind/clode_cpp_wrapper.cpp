@@ -391,6 +391,18 @@ public:
         check(clode_get_step_counts(sim_, v.data(), v.size()));
         return v;
     }
+    std::vector<unsigned long long> get_trial_counts()
+    {
+        std::vector<unsigned long long> v((size_t)n_pts());
+        check(clode_get_trial_counts(sim_, v.data(), v.size()));
+        return v;
+    }
+    py::array_t<unsigned long long> get_rng_state()
+    {
+        py::array_t<unsigned long long> v((py::ssize_t)(2 * (size_t)n_pts()));
+        check(clode_get_rng_state(sim_, v.mutable_data(), (size_t)v.size()));
+        return v;
+    }
 
 protected:
     clode_sim *sim_ = nullptr;
@@ -609,7 +621,8 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .value("critical", critical)
         .value("off", off)
         .export_values();
-    py::class_<Logger>(m, "Logger")
+    // bound under the reference's class name (clode/cpp/CLODEpython.cpp:87)
+    py::class_<Logger>(m, "LoggerSingleton")
         .def("set_log_level", &Logger::set_log_level)
         .def("get_log_level", &Logger::get_log_level)
         .def("set_log_pattern", &Logger::set_log_pattern);
@@ -808,7 +821,9 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("last_kernel_ms", &SimulatorBasePy::last_kernel_ms)
         .def("set_step_counter", &SimulatorBasePy::set_step_counter)
         .def("reset_step_counts", &SimulatorBasePy::reset_step_counts)
-        .def("get_step_counts", &SimulatorBasePy::get_step_counts);
+        .def("get_step_counts", &SimulatorBasePy::get_step_counts)
+        .def("get_trial_counts", &SimulatorBasePy::get_trial_counts)
+        .def("get_rng_state", &SimulatorBasePy::get_rng_state);
 
     py::class_<FeatureSimulatorBasePy, SimulatorBasePy>(m, "FeatureSimulatorBase")
         .def(py::init<const ProblemInfoPy &, const std::string &, const std::string &, const clode_observer_params &,

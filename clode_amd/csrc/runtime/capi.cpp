@@ -724,5 +724,27 @@ int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n)
     std::copy(v.begin(), v.end(), out);
     CLODE_CATCH
 }
+int clode_get_trial_counts(clode_sim *sim, unsigned long long *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    std::vector<unsigned long long> v = sim->s->getStepCounts(1);
+    if (n < v.size())
+        throw InvalidArgument("output buffer too small");
+    if (!v.empty())
+        need(out, "output buffer");
+    std::copy(v.begin(), v.end(), out);
+    CLODE_CATCH
+}
+int clode_get_rng_state(clode_sim *sim, unsigned long long *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    std::vector<unsigned long long> v = sim->s->getRngState();
+    if (n < v.size())
+        throw InvalidArgument("output buffer too small");
+    if (!v.empty())
+        need(out, "output buffer");
+    std::copy(v.begin(), v.end(), out);
+    CLODE_CATCH
+}
 
 }  // extern "C"

@@ -107,6 +107,8 @@ private:
 };
 
 std::string compose_source(const ProgramConfig &cfg);
+// toolchain text mixed into every program's cache key
+std::string toolchain_identity();
 std::vector<std::string> compile_options(const ProgramConfig &cfg);
 std::shared_ptr<Program> get_program(const ProgramConfig &cfg, const std::string &cache_dir);
 std::shared_ptr<Program> get_program_source(const std::string &source, const std::vector<std::string> &options,

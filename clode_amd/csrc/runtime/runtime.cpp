@@ -16,6 +16,8 @@
 #include <iostream>
 #include <cstdlib>
 #include <sstream>
+#include <dlfcn.h>
+#include <hip/hip_version.h>
 #include <sys/stat.h>
 #include <unistd.h>
 
@@ -402,17 +404,25 @@ static std::string cache_path(const std::string &dir, uint64_t key)
 // header or checksum does not match is ignored and rebuilt.
 static bool read_cache(const std::string &path, uint64_t key, std::vector<char> &code)
 {
-    std::ifstream f(path, std::ios::binary);
+    std::ifstream f(path, std::ios::binary | std::ios::ate);
     if (!f)
         return false;
+    const std::streamoff fileSize = f.tellg();
+    f.seekg(0);
     char magic[8];
     uint64_t k = 0, n = 0, sum = 0;
     f.read(magic, 8);
     f.read((char *)&k, 8);
     f.read((char *)&n, 8);
     f.read((char *)&sum, 8);
-    if (!f || std::memcmp(magic, "CLODECO2", 8) != 0 || k != key || n == 0 || n > (1ull << 30))
+    if (!f || std::memcmp(magic, "CLODECO2", 8) != 0 || k != key)
         return false;
+    // the header's size must account for the whole file before anything is
+    // allocated: a truncated file or a damaged size field is rebuilt
+    if (n == 0 || fileSize < 32 || (uint64_t)fileSize != 32 + n) {
+        log_msg(LOG_WARN, "ignoring damaged cached code object " + path);
+        return false;
+    }
     std::vector<char> body(n);
     f.read(body.data(), (std::streamsize)n);
     if (!f || fnv1a(body.data(), n) != sum) {
@@ -448,6 +458,34 @@ static void write_cache(const std::string &dir, const std::string &path, uint64_
     std::rename(tmp.c_str(), path.c_str());
 }
 
+// Identity of the toolchain that builds code objects, part of every program
+// key: the HIP release the library was built against (major.minor.patch and
+// git hash), the version the HIP runtime reports, and the path, size and
+// modification time of the hipRTC library actually loaded.  hipRTC's own
+// version query returns a fixed API version, so it cannot tell two compilers
+// apart; a ROCm upgrade or another install changes at least one of these.
+// CLODE_TOOLCHAIN_ID (environment) is appended verbatim, so a user can force
+// every cached code object to be rebuilt.
+std::string toolchain_identity()
+{
+    std::ostringstream s;
+    s << "hip " << HIP_VERSION_MAJOR << "." << HIP_VERSION_MINOR << "." << HIP_VERSION_PATCH << " "
+      << HIP_VERSION_GITHASH;
+    int rv = 0;
+    if (hipRuntimeGetVersion(&rv) == hipSuccess)
+        s << " runtime " << rv;
+    Dl_info info;
+    if (dladdr(reinterpret_cast<void *>(&hiprtcCompileProgram), &info) && info.dli_fname) {
+        s << " hiprtc " << info.dli_fname;
+        struct stat st;
+        if (stat(info.dli_fname, &st) == 0)
+            s << " " << (long long)st.st_size << " " << (long long)st.st_mtime;
+    }
+    if (const char *e = std::getenv("CLODE_TOOLCHAIN_ID"))
+        s << " id " << e;
+    return s.str();
+}
+
 static void rtc_check(hiprtcResult r, const char *what)
 {
     if (r != HIPRTC_SUCCESS)
@@ -471,9 +509,7 @@ std::shared_ptr<Program> get_program_source(const std::string &source, const std
     for (const auto &h : kDeviceSources)
         keytext += h.text;
     // a different compiler must not reuse code objects cached by another one
-    int rtc_major = 0, rtc_minor = 0;
-    if (hiprtcVersion(&rtc_major, &rtc_minor) == HIPRTC_SUCCESS)
-        keytext += "\nhiprtc " + std::to_string(rtc_major) + "." + std::to_string(rtc_minor);
+    keytext += "\n" + toolchain_identity();
     const uint64_t key = fnv1a(keytext);
 
     std::lock_guard<std::mutex> lk(g_prog_mu);

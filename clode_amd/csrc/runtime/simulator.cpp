@@ -476,8 +476,8 @@ ClodeArgs Simulator::makeArgs()
     a.nStored = d_nStored.as<int>();
     a.tspan_m = memberTspan ? d_tspan_m.ptr : nullptr;
     if (stepCounter) {
-        if (!d_steps.ptr || d_steps.bytes < (size_t)nPts * 8) {
-            d_steps.allocate(device, (size_t)nPts * 8);
+        if (!d_steps.ptr || d_steps.bytes != (size_t)nPts * 16) {  // [2][nPts]: steps, trials
+            d_steps.allocate(device, (size_t)nPts * 16);
             resetStepCounts();
         }
         a.steps = d_steps.as<unsigned long long>();
@@ -561,14 +561,26 @@ void Simulator::resetStepCounts()
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
 }
 
-std::vector<unsigned long long> Simulator::getStepCounts()
+std::vector<unsigned long long> Simulator::getStepCounts(int row)
 {
     std::vector<unsigned long long> out(nPts > 0 ? nPts : 0, 0ull);
-    if (!d_steps.ptr || out.empty())
+    if (!d_steps.ptr || out.empty() || d_steps.bytes < (size_t)nPts * 16)
         return out;
     ensureStream();
-    hip_check(hipMemcpyAsync(out.data(), d_steps.ptr, out.size() * 8, hipMemcpyDeviceToHost, stream),
+    hip_check(hipMemcpyAsync(out.data(), d_steps.as<unsigned long long>() + (size_t)row * nPts, out.size() * 8,
+                             hipMemcpyDeviceToHost, stream),
               "hipMemcpy D2H");
+    hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    return out;
+}
+
+std::vector<unsigned long long> Simulator::getRngState()
+{
+    std::vector<unsigned long long> out(rngHost.size(), 0ull);
+    if (out.empty() || !d_rng.ptr)
+        return out;
+    ensureStream();
+    hip_check(hipMemcpyAsync(out.data(), d_rng.ptr, out.size() * 8, hipMemcpyDeviceToHost, stream), "hipMemcpy D2H");
     hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
     return out;
 }

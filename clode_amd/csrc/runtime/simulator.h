@@ -99,7 +99,9 @@ public:
 
     void setStepCounter(bool on);
     void resetStepCounts();
-    std::vector<unsigned long long> getStepCounts();
+    // row 0: accepted steps per member; row 1: step trials (a step and its rejected trials)
+    std::vector<unsigned long long> getStepCounts(int row = 0);
+    std::vector<unsigned long long> getRngState();
 
 private:
     friend class Batch;

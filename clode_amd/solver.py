@@ -431,3 +431,9 @@ class Simulator:
     def get_step_counts(self) -> np.ndarray:
         return np.array(self._integrator.get_step_counts(), dtype=np.uint64).reshape(self._ensemble_shape,
                                                                                      order="F")
+
+    def get_trial_counts(self) -> np.ndarray:
+        """Step trials per member over the counted launches: the accepted
+        steps plus the adaptive controller's rejected trials."""
+        return np.array(self._integrator.get_trial_counts(), dtype=np.uint64).reshape(self._ensemble_shape,
+                                                                                      order="F")

@@ -209,6 +209,15 @@ int clode_compile(clode_sim *sim);
 int clode_set_step_counter(clode_sim *sim, int enable);
 int clode_reset_step_counts(clode_sim *sim);
 int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n);
+/* Per-member count of step trials over the same launches: one per stepper
+ * call, i.e. the accepted steps plus the rejected trials of the adaptive
+ * controller. */
+int clode_get_trial_counts(clode_sim *sim, unsigned long long *out, size_t n);
+/* The members' noise-generator state, [2][nPts] words (s0 row, then s1): the
+ * reference keeps it on the device only (clode/cpp/CLODE.cpp:420-464 writes
+ * it, the kernels advance it); exposed so that callers can check or save the
+ * stream position. */
+int clode_get_rng_state(clode_sim *sim, unsigned long long *out, size_t n);
 
 /* ---- batches: several simulators in one work-queue launch ------------- */
 /* No reference counterpart (the reference runs one right-hand side per

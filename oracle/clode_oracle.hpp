@@ -47,6 +47,36 @@
 
 namespace oracle {
 
+// Elementary operations are called unqualified, so that a real type other
+// than float / double (the operation-counting type of flop_count.cpp) brings
+// its own overloads by argument-dependent lookup.
+using std::fabs;
+using std::fma;
+using std::fmax;
+using std::fmin;
+using std::nextafter;
+using std::sqrt;
+
+// Hooks at the start of a step and at the end of every step trial (0
+// rejected, 1 accepted, 2 failed at hmin), and a marker for values that vary
+// during the integration; all empty except in the operation-counting build.
+#ifndef CLODE_ORACLE_TRIAL_BEGIN
+#define CLODE_ORACLE_TRIAL_BEGIN()
+#endif
+// step trials of the calling thread (one per stepper call, whatever its outcome)
+inline unsigned long long &trial_count()
+{
+    static thread_local unsigned long long n = 0;
+    return n;
+}
+#ifndef CLODE_ORACLE_TRIAL_END
+#define CLODE_ORACLE_TRIAL_END(outcome) (++oracle::trial_count())
+#endif
+template <class R>
+inline void mark_varying(R &)
+{
+}
+
 enum { ST_EULER = 1, ST_HEUN = 2, ST_RK4 = 3, ST_BS23 = 4, ST_DOPRI5 = 5, ST_SEULER = 6 };
 enum { OB_NONE = 0, OB_THRESH2 = 1, OB_LOCALMAX = 2, OB_NHOOD2 = 3, OB_BASIC = 4, OB_BASICALL = 5, OB_NHOOD1 = 6 };
 
@@ -119,7 +149,7 @@ inline R randn(RngData<R> &rd)
             x2 = OC(2.0) * rand01<R>(rd.state) - OC(1.0);
             w = x1 * x1 + x2 * x2;
         } while (w >= OC(1.0) && ++guard < 4096);
-        w = std::sqrt((-OC(2.0) * cm::log(w)) / w);
+        w = sqrt((-OC(2.0) * cm::log(w)) / w);
         y1 = x1 * w;
         rd.randnLast = x2 * w;
         rd.randnUselast = true;
@@ -167,7 +197,7 @@ struct Integrator {
         R th = *ti + dt;
         Buf<R> tmp, k2;
         for (int k = 0; k < pb.nVar; k++)
-            tmp[k] = std::fma(dt, k1[k], xi[k]);
+            tmp[k] = fma(dt, k1[k], xi[k]);
         rhs(th, tmp.data(), p, k2.data(), aux, wi);
         for (int k = 0; k < pb.nVar; k++)
             xi[k] += dt * OC(0.5) * (k1[k] + k2[k]);
@@ -198,6 +228,7 @@ struct Integrator {
     int fixed_stepper(R *ti, R *xi, R *k1, const R *p, const SolverParamsT<R> &sp, R *dt, R t1, R *aux, R *wi,
                       RngData<R> &rd) const
     {
+        CLODE_ORACLE_TRIAL_BEGIN();
         if (stepper == ST_EULER || stepper == ST_SEULER)
             euler_step(ti, xi, k1, p, *dt, aux, wi);
         else if (stepper == ST_HEUN)
@@ -206,8 +237,9 @@ struct Integrator {
             rk4_step(ti, xi, k1, p, *dt, aux, wi);
         if (stepper == ST_SEULER)
             for (int j = 0; j < pb.nWiener; ++j)
-                wi[j] = randn(rd) / std::sqrt(*dt);
+                wi[j] = randn(rd) / sqrt(*dt);
         rhs(*ti, xi, p, k1, aux, wi);
+        CLODE_ORACLE_TRIAL_END(1);
         return 0;
     }
 
@@ -287,6 +319,7 @@ struct Integrator {
     int adaptive_stepper(R *ti, R *xi, R *k1, const R *p, const SolverParamsT<R> &sp, R *dt, R t1, R *aux, R *wi,
                          RngData<R> &rd) const
     {
+        CLODE_ORACLE_TRIAL_BEGIN();
         const int N = pb.nVar;
         const bool bs = stepper == ST_BS23;
         const R order = bs ? OC(2.0) : OC(4.0);
@@ -297,54 +330,66 @@ struct Integrator {
         Buf<R> err, newxi, newk1;
         R newDt = *dt;
         R threshold = sp.abstol / sp.reltol;
-        R hmin = OC(16.0) * std::fabs(std::fabs(std::nextafter(*ti, OC(1.1) * t1)) - *ti);
+        R hmin = OC(16.0) * fabs(fabs(nextafter(*ti, OC(1.1) * t1)) - *ti);
         bool noFailedSteps = true;
-        int attempts = 0;  // same bound as the library (100 attempts -> hmin failure)
+        // The reference's loop ends at hmin.  A rejected trial is followed by
+        // a strictly smaller clamped step until the clamp reaches hmin, so a
+        // trial equal to the previous one happens only at hmin when t + hmin
+        // rounds to a longer step; the reference would repeat it forever, and
+        // it fails at hmin here (stall).  4096 trials bound the loop in any
+        // case; a sequence that ends in the reference needs at most ~2300.
+        R lastTrial = OC(0.0);
+        int attempts = 0;
         while (true) {
-            if (++attempts > 100) {
+            newDt = fmin(fmax(newDt, hmin), sp.dtmax);  // OpenCL clamp
+            if ((!noFailedSteps && newDt == lastTrial) || ++attempts > 4096) {
                 *dt = hmin;
+                CLODE_ORACLE_TRIAL_END(2);
                 return -1;
             }
+            lastTrial = newDt;
             tNew = *ti;
             for (int j = 0; j < N; j++) {
                 newxi[j] = xi[j];
                 newk1[j] = k1[j];
             }
-            newDt = std::fmin(std::fmax(newDt, hmin), sp.dtmax);  // OpenCL clamp
             if (bs)
                 newDt = bs23_step(&tNew, newxi.data(), newk1.data(), p, newDt, aux, err.data(), wi);
             else
                 newDt = dopri5_step(&tNew, newxi.data(), newk1.data(), p, newDt, aux, err.data(), wi);
             for (int j = 0; j < N; j++)
-                err[j] /= std::fmax(std::fmax(std::fabs(xi[j]), std::fabs(newxi[j])), threshold);
+                err[j] /= fmax(fmax(fabs(xi[j]), fabs(newxi[j])), threshold);
             normErr = OC(0.0);
             for (int k = 0; k < N; k++)
-                normErr = std::fmax(std::fabs(err[k]), normErr);
+                normErr = fmax(fabs(err[k]), normErr);
             if (normErr > sp.reltol) {
                 if (newDt <= hmin) {
                     *dt = hmin;
+                    CLODE_ORACLE_TRIAL_END(2);
                     return -1;
                 }
                 if (noFailedSteps) {
                     noFailedSteps = false;
-                    newDt *= std::fmax(maxShrink, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
+                    newDt *= fmax(maxShrink, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
                 } else {
                     newDt *= OC(0.5);
                 }
+                CLODE_ORACLE_TRIAL_END(0);
             } else {
                 break;
             }
         }
         if (noFailedSteps)
-            newDt *= std::fmin(maxGrow, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
-        newDt = std::fmin(newDt, t1 - tNew);
-        newDt = std::fmin(std::fmax(newDt, hmin), sp.dtmax);
+            newDt *= fmin(maxGrow, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
+        newDt = fmin(newDt, t1 - tNew);
+        newDt = fmin(fmax(newDt, hmin), sp.dtmax);
         *dt = newDt;
         *ti = tNew;
         for (int j = 0; j < N; j++) {
             xi[j] = newxi[j];
             k1[j] = newk1[j];
         }
+        CLODE_ORACLE_TRIAL_END(1);
         return 0;
     }
 
@@ -470,14 +515,14 @@ struct Observer {
     void warmup(R ti, const R *xi, const R *dxi, ObserverData<R> &od) const
     {
         if (kind == OB_THRESH2) {  // :197-203
-            od.xGlobalMax = std::fmax(od.xGlobalMax, xi[op.eVarIx]);
-            od.xGlobalMin = std::fmin(od.xGlobalMin, xi[op.eVarIx]);
-            od.dxGlobalMax = std::fmax(od.dxGlobalMax, dxi[op.eVarIx]);
-            od.dxGlobalMin = std::fmin(od.dxGlobalMin, dxi[op.eVarIx]);
+            od.xGlobalMax = fmax(od.xGlobalMax, xi[op.eVarIx]);
+            od.xGlobalMin = fmin(od.xGlobalMin, xi[op.eVarIx]);
+            od.dxGlobalMax = fmax(od.dxGlobalMax, dxi[op.eVarIx]);
+            od.dxGlobalMin = fmin(od.dxGlobalMin, dxi[op.eVarIx]);
         } else if (kind == OB_NHOOD2) {  // :142-147
             for (int j = 0; j < N; ++j) {
-                od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
-                od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+                od.xTrajectoryMax[j] = fmax(xi[j], od.xTrajectoryMax[j]);
+                od.xTrajectoryMin[j] = fmin(xi[j], od.xTrajectoryMin[j]);
             }
         }
     }
@@ -508,15 +553,15 @@ struct Observer {
     void extents(R ti, const R *xi, const R *dxi, const R *auxi, ObserverData<R> &od, R dt, R elapsedTime) const
     {
         for (int j = 0; j < N; ++j) {
-            od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
-            od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+            od.xTrajectoryMax[j] = fmax(xi[j], od.xTrajectoryMax[j]);
+            od.xTrajectoryMin[j] = fmin(xi[j], od.xTrajectoryMin[j]);
             od.xTrajectoryMean[j] = runningMeanTime(od.xTrajectoryMean[j], xi[j], dt, elapsedTime);
-            od.dxTrajectoryMax[j] = std::fmax(dxi[j], od.dxTrajectoryMax[j]);
-            od.dxTrajectoryMin[j] = std::fmin(dxi[j], od.dxTrajectoryMin[j]);
+            od.dxTrajectoryMax[j] = fmax(dxi[j], od.dxTrajectoryMax[j]);
+            od.dxTrajectoryMin[j] = fmin(dxi[j], od.dxTrajectoryMin[j]);
         }
         for (int j = 0; j < A; ++j) {
-            od.auxTrajectoryMax[j] = std::fmax(auxi[j], od.auxTrajectoryMax[j]);
-            od.auxTrajectoryMin[j] = std::fmin(auxi[j], od.auxTrajectoryMin[j]);
+            od.auxTrajectoryMax[j] = fmax(auxi[j], od.auxTrajectoryMax[j]);
+            od.auxTrajectoryMin[j] = fmin(auxi[j], od.auxTrajectoryMin[j]);
             od.auxTrajectoryMean[j] = runningMeanTime(od.auxTrajectoryMean[j], auxi[j], dt, elapsedTime);
         }
     }
@@ -544,8 +589,8 @@ struct Observer {
         R elapsedTime = ti - od.t_start;
         const unsigned f = op.fVarIx, e = op.eVarIx;
         if (kind == OB_THRESH2) {  // :308-407
-            od.stepDt[0] = std::fmax(dt, od.stepDt[0]);
-            od.stepDt[1] = std::fmin(dt, od.stepDt[1]);
+            od.stepDt[0] = fmax(dt, od.stepDt[0]);
+            od.stepDt[1] = fmin(dt, od.stepDt[1]);
             runningMean(&od.stepDt[2], dt, od.stepcount);
             extents(ti, xi, dxi, auxi, od, dt, elapsedTime);
             if (od.stepcount > 1) {
@@ -623,33 +668,33 @@ struct Observer {
             for (int j = 0; j < N; ++j) {
                 if (kind == OB_BASIC && j != (int)f)
                     continue;
-                od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
-                od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+                od.xTrajectoryMax[j] = fmax(xi[j], od.xTrajectoryMax[j]);
+                od.xTrajectoryMin[j] = fmin(xi[j], od.xTrajectoryMin[j]);
                 od.xTrajectoryMean[j] = runningMeanTime(od.xTrajectoryMean[j], xi[j], dtb, el);
-                od.dxTrajectoryMax[j] = std::fmax(dxi[j], od.dxTrajectoryMax[j]);
-                od.dxTrajectoryMin[j] = std::fmin(dxi[j], od.dxTrajectoryMin[j]);
+                od.dxTrajectoryMax[j] = fmax(dxi[j], od.dxTrajectoryMax[j]);
+                od.dxTrajectoryMin[j] = fmin(dxi[j], od.dxTrajectoryMin[j]);
             }
             if (kind == OB_BASICALL)
                 for (int j = 0; j < A; ++j) {
-                    od.auxTrajectoryMax[j] = std::fmax(auxi[j], od.auxTrajectoryMax[j]);
-                    od.auxTrajectoryMin[j] = std::fmin(auxi[j], od.auxTrajectoryMin[j]);
+                    od.auxTrajectoryMax[j] = fmax(auxi[j], od.auxTrajectoryMax[j]);
+                    od.auxTrajectoryMin[j] = fmin(auxi[j], od.auxTrajectoryMin[j]);
                     od.auxTrajectoryMean[j] = runningMeanTime(od.auxTrajectoryMean[j], auxi[j], dtb, el);
                 }
         } else if (kind == OB_NHOOD1) {  // observer_neighborhood_1.clh:203-276
-            od.stepDt[0] = std::fmax(dt, od.stepDt[0]);
-            od.stepDt[1] = std::fmin(dt, od.stepDt[1]);
+            od.stepDt[0] = fmax(dt, od.stepDt[0]);
+            od.stepDt[1] = fmin(dt, od.stepDt[1]);
             runningMean(&od.stepDt[2], dt, od.stepcount);
             // step-count means, not time means
             for (int j = 0; j < N; ++j) {
-                od.xTrajectoryMax[j] = std::fmax(xi[j], od.xTrajectoryMax[j]);
-                od.xTrajectoryMin[j] = std::fmin(xi[j], od.xTrajectoryMin[j]);
+                od.xTrajectoryMax[j] = fmax(xi[j], od.xTrajectoryMax[j]);
+                od.xTrajectoryMin[j] = fmin(xi[j], od.xTrajectoryMin[j]);
                 runningMean(&od.xTrajectoryMean[j], xi[j], od.stepcount);
-                od.dxTrajectoryMax[j] = std::fmax(dxi[j], od.dxTrajectoryMax[j]);
-                od.dxTrajectoryMin[j] = std::fmin(dxi[j], od.dxTrajectoryMin[j]);
+                od.dxTrajectoryMax[j] = fmax(dxi[j], od.dxTrajectoryMax[j]);
+                od.dxTrajectoryMin[j] = fmin(dxi[j], od.dxTrajectoryMin[j]);
             }
             for (int j = 0; j < A; ++j) {
-                od.auxTrajectoryMax[j] = std::fmax(auxi[j], od.auxTrajectoryMax[j]);
-                od.auxTrajectoryMin[j] = std::fmin(auxi[j], od.auxTrajectoryMin[j]);
+                od.auxTrajectoryMax[j] = fmax(auxi[j], od.auxTrajectoryMax[j]);
+                od.auxTrajectoryMin[j] = fmin(auxi[j], od.auxTrajectoryMin[j]);
                 runningMean(&od.auxTrajectoryMean[j], auxi[j], od.stepcount);
             }
             if (od.stepcount > 1) {
@@ -673,8 +718,8 @@ struct Observer {
                 }
             }
         } else if (kind == OB_NHOOD2) {  // :208-268
-            od.stepDt[0] = std::fmax(dt, od.stepDt[0]);
-            od.stepDt[1] = std::fmin(dt, od.stepDt[1]);
+            od.stepDt[0] = fmax(dt, od.stepDt[0]);
+            od.stepDt[1] = fmin(dt, od.stepDt[1]);
             runningMean(&od.stepDt[2], dt, od.stepcount);
             extents(ti, xi, dxi, auxi, od, dt, elapsedTime);
             if (od.stepcount < 2)
@@ -718,11 +763,11 @@ struct Observer {
             unsigned int lastInNhood = od.isInNhood;
             Buf<R> d;
             for (int j = 0; j < N; ++j)
-                d[j] = std::fabs(xi[j] - od.x0[j]) / (od.xTrajectoryMax[j] - od.xTrajectoryMin[j]);
+                d[j] = fabs(xi[j] - od.x0[j]) / (od.xTrajectoryMax[j] - od.xTrajectoryMin[j]);
             R result = 0;
             for (int k = 0; k < N; k++)
                 result += d[k] * d[k];
-            od.isInNhood = std::sqrt(result) <= op.nHoodRadius;
+            od.isInNhood = sqrt(result) <= op.nHoodRadius;
             return od.isInNhood && !lastInNhood;
         } else {  // nhood2 :157-175
             if (od.stepcount < 2 || !od.foundX0)
@@ -734,7 +779,7 @@ struct Observer {
             R result = 0;
             for (int k = 0; k < N; k++)
                 result += d[k] * d[k];
-            od.isInNhood = std::sqrt(result) < op.nHoodRadius;
+            od.isInNhood = sqrt(result) < op.nHoodRadius;
             return lastInNhood && !od.isInNhood;
         }
     }
@@ -748,28 +793,28 @@ struct Observer {
             tThisEvent = ti;
             if (od.eventcount > 1) {
                 int nPeriods = od.eventcount - 1;
-                od.nMaxima[0] = std::fmax((R)od.thisNMaxima, od.nMaxima[0]);
-                od.nMaxima[1] = std::fmin((R)od.thisNMaxima, od.nMaxima[1]);
+                od.nMaxima[0] = fmax((R)od.thisNMaxima, od.nMaxima[0]);
+                od.nMaxima[1] = fmin((R)od.thisNMaxima, od.nMaxima[1]);
                 runningMean(&od.nMaxima[2], (R)od.thisNMaxima, nPeriods);
                 R thisPeriod = tThisEvent - od.tLastEvent;
-                od.period[0] = std::fmax(thisPeriod, od.period[0]);
-                od.period[1] = std::fmin(thisPeriod, od.period[1]);
+                od.period[0] = fmax(thisPeriod, od.period[0]);
+                od.period[1] = fmin(thisPeriod, od.period[1]);
                 runningMean(&od.period[2], thisPeriod, nPeriods);
                 R thisUpDuration = od.tThisDown - od.tLastEvent;
-                od.upDuration[0] = std::fmax(thisUpDuration, od.upDuration[0]);
-                od.upDuration[1] = std::fmin(thisUpDuration, od.upDuration[1]);
+                od.upDuration[0] = fmax(thisUpDuration, od.upDuration[0]);
+                od.upDuration[1] = fmin(thisUpDuration, od.upDuration[1]);
                 runningMean(&od.upDuration[2], thisUpDuration, nPeriods);
                 R thisDownDuration = tThisEvent - od.tThisDown;
-                od.downDuration[0] = std::fmax(thisDownDuration, od.downDuration[0]);
-                od.downDuration[1] = std::fmin(thisDownDuration, od.downDuration[1]);
+                od.downDuration[0] = fmax(thisDownDuration, od.downDuration[0]);
+                od.downDuration[1] = fmin(thisDownDuration, od.downDuration[1]);
                 runningMean(&od.downDuration[2], thisDownDuration, nPeriods);
                 R thisduty = thisUpDuration / thisPeriod;
-                od.duty[0] = std::fmax(thisduty, od.duty[0]);
-                od.duty[1] = std::fmin(thisduty, od.duty[1]);
+                od.duty[0] = fmax(thisduty, od.duty[0]);
+                od.duty[1] = fmin(thisduty, od.duty[1]);
                 runningMean(&od.duty[2], thisduty, nPeriods);
                 R thisActiveDip = od.fVarDownstateMean - od.xLastMin;
-                od.activeDip[0] = std::fmax(thisActiveDip, od.activeDip[0]);
-                od.activeDip[1] = std::fmin(thisActiveDip, od.activeDip[1]);
+                od.activeDip[0] = fmax(thisActiveDip, od.activeDip[0]);
+                od.activeDip[1] = fmin(thisActiveDip, od.activeDip[1]);
                 runningMean(&od.activeDip[2], thisActiveDip, nPeriods);
             }
             if (od.eventcount <= (unsigned)S)
@@ -794,12 +839,12 @@ struct Observer {
             ++od.eventcount;
             if (od.eventcount > 1) {
                 R thisIMI = tThisMax - od.tLastMax;
-                od.IMI[0] = std::fmax(thisIMI, od.IMI[0]);
-                od.IMI[1] = std::fmin(thisIMI, od.IMI[1]);
+                od.IMI[0] = fmax(thisIMI, od.IMI[0]);
+                od.IMI[1] = fmin(thisIMI, od.IMI[1]);
                 runningMean(&od.IMI[2], thisIMI, od.eventcount - 1);
                 R thisAmp = xThisMax - od.xLastMin;
-                od.amp[0] = std::fmax(thisAmp, od.amp[0]);
-                od.amp[1] = std::fmin(thisAmp, od.amp[1]);
+                od.amp[0] = fmax(thisAmp, od.amp[0]);
+                od.amp[1] = fmin(thisAmp, od.amp[1]);
                 runningMean(&od.amp[2], thisAmp, od.eventcount - 1);
             }
             od.tLastMax = tThisMax;
@@ -814,12 +859,12 @@ struct Observer {
             ++od.eventcount;
             R tThisEvent = ti;
             if (od.eventcount > 1) {
-                od.nMaxima[0] = std::fmax((R)od.thisNMaxima, od.nMaxima[0]);
-                od.nMaxima[1] = std::fmin((R)od.thisNMaxima, od.nMaxima[1]);
+                od.nMaxima[0] = fmax((R)od.thisNMaxima, od.nMaxima[0]);
+                od.nMaxima[1] = fmin((R)od.thisNMaxima, od.nMaxima[1]);
                 runningMean(&od.nMaxima[2], (R)od.thisNMaxima, od.eventcount - 1);
                 R thisPeriod = tThisEvent - od.tLastEvent;
-                od.period[0] = std::fmax(thisPeriod, od.period[0]);
-                od.period[1] = std::fmin(thisPeriod, od.period[1]);
+                od.period[0] = fmax(thisPeriod, od.period[0]);
+                od.period[1] = fmin(thisPeriod, od.period[1]);
                 runningMean(&od.period[2], thisPeriod, od.eventcount - 1);
             }
             od.tLastEvent = tThisEvent;
@@ -829,12 +874,12 @@ struct Observer {
             ++od.eventcount;
             R tThisEvent = ti;
             if (od.eventcount > 1) {
-                od.nMaxima[0] = std::fmax((R)od.thisNMaxima, od.nMaxima[0]);
-                od.nMaxima[1] = std::fmin((R)od.thisNMaxima, od.nMaxima[1]);
+                od.nMaxima[0] = fmax((R)od.thisNMaxima, od.nMaxima[0]);
+                od.nMaxima[1] = fmin((R)od.thisNMaxima, od.nMaxima[1]);
                 runningMean(&od.nMaxima[2], (R)od.thisNMaxima, od.eventcount - 1);
                 R thisPeriod = tThisEvent - od.tLastEvent;
-                od.period[0] = std::fmax(thisPeriod, od.period[0]);
-                od.period[1] = std::fmin(thisPeriod, od.period[1]);
+                od.period[0] = fmax(thisPeriod, od.period[0]);
+                od.period[1] = fmin(thisPeriod, od.period[1]);
                 runningMean(&od.period[2], thisPeriod, od.eventcount - 1);
             }
             od.tLastEvent = tThisEvent;
@@ -972,8 +1017,15 @@ struct Observer {
     }
 
     // finalizeObserverData: shift stored times by -(tf - t0)
+    // (observer_threshold_2.clh:458-470, observer_local_maximum.clh:319-328,
+    // observer_neighborhood_2.clh:305-313).  CLODE_ORACLE_MUTANT_NO_TIME_SHIFT
+    // builds a deliberately wrong checker without the shift, which the
+    // continuation tests must tell apart from this one.
     void finalizeObserverData(R ti, R t0, ObserverData<R> &od) const
     {
+#ifdef CLODE_ORACLE_MUTANT_NO_TIME_SHIFT
+        return;
+#endif
         R T = ti - t0;
         od.t_start -= T;
         if (kind == OB_THRESH2) {

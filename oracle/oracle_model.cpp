@@ -23,7 +23,18 @@
 
 #define CM_FN inline
 #include "../clode_amd/csrc/device/cm_math.h"
+#ifdef CLODE_ORACLE_COUNTING
+#include "flop_count.hpp"
+#endif
 #include "clode_oracle.hpp"
+#include "oracle_ensemble.hpp"
+#ifdef CLODE_ORACLE_COUNTING
+template <>
+struct oracle::Consts<fc::Counted> {
+    static fc::Counted big() { return fc::Counted(1.7976931348623157e308); }
+    static fc::Counted c(double d, float) { return fc::Counted(d); }
+};
+#endif
 
 // ---- OpenCL C built-ins for the right-hand side, host versions -----------
 static inline double sinpi(double x) { return std::sin(3.141592653589793238462643383279502884 * x); }
@@ -106,6 +117,22 @@ typedef double realtype;
 #undef BIG_REAL
 }  // namespace rhs64
 
+#ifdef CLODE_ORACLE_COUNTING
+// the right-hand side on the operation-counting type (flop_count.hpp)
+namespace rhsC {
+typedef fc::Counted realtype;
+#define RCONST(x) (x)
+#define ZERO 0.0
+#define ONE 1.0
+#define BIG_REAL 1.7976931348623157e308
+#include CLODE_ORACLE_RHS
+#undef RCONST
+#undef ZERO
+#undef ONE
+#undef BIG_REAL
+}  // namespace rhsC
+#endif
+
 #undef exp
 #undef log
 #undef pow
@@ -113,306 +140,44 @@ typedef double realtype;
 
 using namespace oracle;
 
-namespace {
-
-template <class R>
-struct RhsOf;
 template <>
-struct RhsOf<float> {
+struct oracle_drv::RhsOf<float> {
     static void call(const float t, const float *x, const float *p, float *dx, float *aux, const float *w)
     {
         rhs32::getRHS(t, x, p, dx, aux, w);
     }
 };
 template <>
-struct RhsOf<double> {
+struct oracle_drv::RhsOf<double> {
     static void call(const double t, const double *x, const double *p, double *dx, double *aux, const double *w)
     {
         rhs64::getRHS(t, x, p, dx, aux, w);
     }
 };
 
-struct Config {
-    int kind, prec, stepper, observer, nVar, nPar, nAux, nWiener, nStore, nthreads;
-};
-
-template <class R>
-struct Ensemble {
-    Config cfg;
-    Problem<R> pb;
-    int nPts = 0;
-    std::vector<R> x0, xf, pars, dt, tf;
-    std::vector<uint64_t> rng;
-    R t0 = 0, t1 = 1000;
-    double tspan_d[2] = {0, 1000};
-    std::vector<double> tspanM;  // optional per-member spans [2][nPts]: row 0 start, row 1 end
-    R mt0(int i) const { return tspanM.empty() ? t0 : (R)tspanM[i]; }
-    R mt1(int i) const { return tspanM.empty() ? t1 : (R)tspanM[(size_t)nPts + i]; }
-    SolverParamsT<R> sp{};
-    double sp_dt_d = 0.1;
-    ObserverParamsT<R> op{};
-    std::vector<ObserverData<R>> od;
-    bool obsInit = false;
-    std::vector<R> F;
-    std::vector<R> tt, tx, tdx, taux;
-    std::vector<int> nStored;
-    unsigned int storeAlloc = 0;
-    unsigned long long lastSteps = 0;  // member-steps of the last pass
-    unsigned long long warmSteps = 0;  // member-steps of the observer warm-up run by the last features()
-
-    explicit Ensemble(const Config &c) : cfg(c)
+#ifdef CLODE_ORACLE_COUNTING
+template <>
+struct oracle_drv::RhsOf<fc::Counted> {
+    static void call(const fc::Counted t, const fc::Counted *x, const fc::Counted *p, fc::Counted *dx,
+                     fc::Counted *aux, const fc::Counted *w)
     {
-        pb.nVar = c.nVar;
-        pb.nPar = c.nPar;
-        pb.nAux = c.nAux;
-        pb.nWiener = c.nWiener;
-        pb.rhs = &RhsOf<R>::call;
-        sp.dt = (R)0.1;
-        sp.dtmax = (R)0.5;
-        sp.abstol = (R)1e-6;
-        sp.reltol = (R)1e-3;
-        sp.max_steps = 1000000;
-        sp.max_store = 1000000;
-        sp.nout = 1;
-    }
-
-    Observer<R> observer() const
-    {
-        Observer<R> o;
-        o.kind = cfg.observer;
-        o.N = cfg.nVar;
-        o.A = cfg.nAux;
-        o.S = cfg.nStore;
-        o.op = op;
-        return o;
-    }
-
-    void setNpts(int n)
-    {
-        if (n == nPts)
-            return;
-        nPts = n;
-        x0.assign((size_t)cfg.nVar * n, 0);
-        xf.assign((size_t)cfg.nVar * n, 0);
-        pars.assign((size_t)cfg.nPar * n, 0);
-        dt.assign(n, (R)sp_dt_d);
-        tf.assign(n, 0);
-        rng.assign(2 * (size_t)n, 0);
-        od.clear();
-        F.clear();
-        obsInit = false;
-        storeAlloc = 0;
-        tspanM.clear();
-    }
-
-    struct Member {
-        std::vector<R> p, x, dx, aux, w;
-        R ti, dt;
-        RngData<R> rd;
-    };
-
-    void load(Member &m, int i) const
-    {
-        const int N = cfg.nVar;
-        m.p.assign(std::max(1, cfg.nPar), 0);
-        m.x.assign(N, 0);
-        m.dx.assign(N, 0);
-        m.aux.assign(std::max(1, cfg.nAux), 0);
-        m.w.assign(std::max(1, cfg.nWiener), 0);
-        m.ti = mt0(i);
-        m.dt = dt[i];
-        for (int j = 0; j < cfg.nPar; ++j)
-            m.p[j] = pars[(size_t)j * nPts + i];
-        for (int j = 0; j < N; ++j)
-            m.x[j] = x0[(size_t)j * nPts + i];
-        m.rd.state[0] = rng[i];
-        m.rd.state[1] = rng[(size_t)nPts + i];
-        m.rd.randnUselast = false;
-        m.rd.randnLast = 0;
-        for (int j = 0; j < cfg.nWiener; ++j)
-            m.w[j] = cfg.stepper == ST_SEULER ? randn(m.rd) / std::sqrt(m.dt) : R(0);
-    }
-
-    void store(const Member &m, int i)
-    {
-        for (int j = 0; j < cfg.nVar; ++j)
-            xf[(size_t)j * nPts + i] = m.x[j];
-        rng[i] = m.rd.state[0];
-        rng[(size_t)nPts + i] = m.rd.state[1];
-        dt[i] = m.dt;
-        tf[i] = m.ti;
-    }
-
-    // transient.cl:9-77
-    unsigned int transientMember(int i)
-    {
-        Integrator<R> ig(pb, cfg.stepper);
-        Member m;
-        load(m, i);
-        pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
-        const R t1 = mt1(i);
-        unsigned int step = 0;
-        while (m.ti <= t1 && step < sp.max_steps) {
-            ++step;
-            ig.step(&m.ti, m.x.data(), m.dx.data(), m.p.data(), sp, &m.dt, t1, m.aux.data(), m.w.data(), m.rd);
-        }
-        store(m, i);
-        return step;
-    }
-
-    // initializeObserver.cl:9-84
-    unsigned int initObserverMember(int i)
-    {
-        Integrator<R> ig(pb, cfg.stepper);
-        Observer<R> ob = observer();
-        Member m;
-        load(m, i);
-        pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
-        ObserverData<R> &d = od[i];
-        ob.init(m.ti, m.x.data(), m.dx.data(), d);
-        const R t0 = mt0(i), t1 = mt1(i);
-        unsigned int step = 0;
-        if (ob.twoPass()) {
-            while (m.ti < t1 && step < sp.max_steps) {
-                ++step;
-                ig.step(&m.ti, m.x.data(), m.dx.data(), m.p.data(), sp, &m.dt, t1, m.aux.data(), m.w.data(), m.rd);
-                ob.warmup(m.ti, m.x.data(), m.dx.data(), d);
-            }
-            m.ti = t0;
-            for (int j = 0; j < cfg.nVar; ++j)
-                m.x[j] = x0[(size_t)j * nPts + i];
-            pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
-        }
-        ob.initEventDetector(m.ti, m.x.data(), m.dx.data(), d);
-        return step;
-    }
-
-    // features.cl:10-106
-    unsigned int featuresMember(int i)
-    {
-        Integrator<R> ig(pb, cfg.stepper);
-        Observer<R> ob = observer();
-        Member m;
-        load(m, i);
-        ObserverData<R> &d = od[i];
-        pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
-        const R t0 = mt0(i), t1 = mt1(i);
-        unsigned int step = 0;
-        while (m.ti <= t1 && step < sp.max_steps) {
-            ++step;
-            ig.step(&m.ti, m.x.data(), m.dx.data(), m.p.data(), sp, &m.dt, t1, m.aux.data(), m.w.data(), m.rd);
-            ob.update(m.ti, m.x.data(), m.dx.data(), m.aux.data(), d);
-            if (ob.eventFunction(m.ti, m.x.data(), m.dx.data(), d))
-                if (ob.computeEventFeatures(m.ti, m.x.data(), m.dx.data(), d))
-                    break;
-        }
-        std::vector<R> f;
-        ob.finalizeFeatures(d, f);
-        for (size_t k = 0; k < f.size(); ++k)
-            F[k * nPts + i] = f[k];
-        ob.finalizeObserverData(m.ti, t0, d);
-        store(m, i);
-        return step;
-    }
-
-    // trajectory.cl:14-112 (never stores index max_store)
-    unsigned int trajectoryMember(int i)
-    {
-        Integrator<R> ig(pb, cfg.stepper);
-        Member m;
-        load(m, i);
-        pb.rhs(m.ti, m.x.data(), m.p.data(), m.dx.data(), m.aux.data(), m.w.data());
-        const size_t n = nPts;
-        const int N = cfg.nVar, A = cfg.nAux;
-        auto put = [&](unsigned int s) {
-            tt[s * n + i] = m.ti;
-            for (int j = 0; j < N; ++j) {
-                tx[(s * N + j) * n + i] = m.x[j];
-                tdx[(s * N + j) * n + i] = m.dx[j];
-            }
-            for (int j = 0; j < A; ++j)
-                taux[(s * A + j) * n + i] = m.aux[j];
-        };
-        const R t1 = mt1(i);
-        unsigned int storeix = 0;
-        if (sp.max_store > 0)  // max_store 0: no sample buffer at all
-            put(storeix);
-        unsigned int step = 0;
-        while (m.ti <= t1 && step < sp.max_steps && storeix < sp.max_store) {
-            ++step;
-            ig.step(&m.ti, m.x.data(), m.dx.data(), m.p.data(), sp, &m.dt, t1, m.aux.data(), m.w.data(), m.rd);
-            if (step % sp.nout == 0) {
-                ++storeix;
-                if (storeix < sp.max_store)
-                    put(storeix);
-            }
-        }
-        nStored[i] = (int)storeix;
-        store(m, i);
-        return step;
-    }
-
-    template <class Fn>
-    void forEach(Fn fn)
-    {
-        unsigned long long total = 0;
-#pragma omp parallel for schedule(dynamic, 1) num_threads(cfg.nthreads > 0 ? cfg.nthreads : 1) reduction(+ : total)
-        for (int i = 0; i < nPts; ++i)
-            total += fn(i);
-        lastSteps = total;
-    }
-
-    void ensureObserver()
-    {
-        Observer<R> ob = observer();
-        size_t fe = (size_t)ob.nFeatures() * nPts;
-        if (F.size() != fe || od.size() != (size_t)nPts) {
-            F.assign(fe, 0);
-            od.assign(nPts, ObserverData<R>());
-            for (auto &d : od)
-                d.resize(cfg.nVar, cfg.nAux, cfg.nStore);
-            obsInit = false;
-        }
-    }
-
-    void initializeObserver()
-    {
-        ensureObserver();
-        forEach([this](int i) { return initObserverMember(i); });
-        obsInit = true;
-    }
-
-    void features(int reinit)
-    {
-        if (reinit == 1)
-            obsInit = false;
-        else if (reinit == 0)
-            obsInit = true;
-        ensureObserver();
-        warmSteps = 0;
-        if (!obsInit) {
-            initializeObserver();
-            warmSteps = lastSteps;
-        }
-        forEach([this](int i) { return featuresMember(i); });
-    }
-
-    void trajectory()
-    {
-        size_t te = (size_t)sp.max_store * nPts;
-        tt.assign(te, 0);
-        tx.assign(te * cfg.nVar, 0);
-        tdx.assign(te * cfg.nVar, 0);
-        taux.assign(cfg.nAux > 0 ? te * cfg.nAux : 1, 0);
-        nStored.assign(nPts, 0);
-        forEach([this](int i) { return trajectoryMember(i); });
+        rhsC::getRHS(t, x, p, dx, aux, w);
     }
 };
+#endif
+
+namespace {
+
+using oracle_drv::Config;
+using oracle_drv::Ensemble;
 
 struct Handle {
     Config cfg;
     Ensemble<float> *f32 = nullptr;
     Ensemble<double> *f64 = nullptr;
+#ifdef CLODE_ORACLE_COUNTING
+    Ensemble<fc::Counted> *cnt = nullptr;  // precision 9: operation counting
+#endif
 };
 
 template <class Fn>
@@ -421,6 +186,10 @@ int with(void *h, Fn fn)
     Handle *H = (Handle *)h;
     if (!H)
         return 1;
+#ifdef CLODE_ORACLE_COUNTING
+    if (H->cnt)
+        return fn(*H->cnt);
+#endif
     if (H->f32)
         return fn(*H->f32);
     return fn(*H->f64);
@@ -443,6 +212,13 @@ void *oracle_create(int kind, int prec, int stepper, int observer, int nVar, int
         return nullptr;
     Handle *H = new Handle;
     H->cfg = Config{kind, prec, stepper, observer, nVar, nPar, nAux, nWiener, nStore, 1};
+#ifdef CLODE_ORACLE_COUNTING
+    if (prec == 9) {
+        H->cnt = new Ensemble<fc::Counted>(H->cfg);
+        fc::reset();
+        return H;
+    }
+#endif
     if (prec == 1)
         H->f32 = new Ensemble<float>(H->cfg);
     else
@@ -457,13 +233,45 @@ void oracle_destroy(void *h)
         return;
     delete H->f32;
     delete H->f64;
+#ifdef CLODE_ORACLE_COUNTING
+    delete H->cnt;
+#endif
     delete H;
 }
+
+#ifdef CLODE_ORACLE_COUNTING
+// Operation counts since the counting ensemble was created, in the layout
+// [phase 4][bucket 4][count 10] then [phase 4][trials 3]: phases transient,
+// warm-up, features, trajectory; buckets rejected, accepted, failed trials
+// and the whole phase; counts add, mul, div, fma, exp, log, pow, pown, sqrt,
+// other transcendental; trials rejected, accepted, failed.
+int oracle_counts(double *out, size_t n)
+{
+    const size_t need = (size_t)fc::N_PHASE * fc::N_BUCKET * fc::N_COUNT + (size_t)fc::N_PHASE * 3;
+    if (!out || n < need)
+        return 1;
+    fc::close_phase();
+    size_t k = 0;
+    for (int p = 0; p < fc::N_PHASE; ++p)
+        for (int b = 0; b < fc::N_BUCKET; ++b)
+            for (int c = 0; c < fc::N_COUNT; ++c)
+                out[k++] = (double)fc::st().bucket[p][b].c[c];
+    for (int p = 0; p < fc::N_PHASE; ++p)
+        for (int t = 0; t < 3; ++t)
+            out[k++] = (double)fc::st().trials[p][t];
+    return 0;
+}
+#endif
 
 int oracle_set_threads(void *h, int n)
 {
     return with(h, [&](auto &e) {
+#ifdef CLODE_ORACLE_COUNTING
+        e.cfg.nthreads = 1;  // one running tally
+        (void)n;
+#else
         e.cfg.nthreads = n;
+#endif
         return 0;
     });
 }
@@ -581,6 +389,7 @@ int oracle_set_dt(void *h, const double *d, size_t n)
 int oracle_transient(void *h)
 {
     return with(h, [&](auto &e) {
+        CLODE_ORACLE_PHASE(0);
         e.forEach([&e](int i) { return e.transientMember(i); });
         return 0;
     });
@@ -638,6 +447,16 @@ unsigned long long oracle_last_steps(void *h)
     unsigned long long s = 0;
     with(h, [&](auto &e) {
         s = e.lastSteps;
+        return 0;
+    });
+    return s;
+}
+
+unsigned long long oracle_last_trials(void *h)
+{
+    unsigned long long s = 0;
+    with(h, [&](auto &e) {
+        s = e.lastTrials;
         return 0;
     });
     return s;

@@ -23,6 +23,7 @@ MODEL_INFO = {
     "linear_drag": (["y1", "y2"], ["m", "w", "k", "H"], ["g1"], 0),
     "harmonic": (["x", "y"], ["omega"], [], 0),
     "decay": (["x"], ["lambda"], [], 0),
+    "two_tone": (["u1", "v1", "u2", "v2", "s"], ["b"], [], 0),
 }
 
 OBS_DEFAULTS = dict(e_var_ix=0, f_var_ix=0, max_event_count=100, max_event_timestamps=0, min_amp=0.0, min_imi=0.0,
@@ -107,6 +108,11 @@ class Pair:
         self.gpu.transient()
         if self.orc:
             self.orc.transient()
+
+    def initialize_observer(self):
+        self.gpu.initialize_observer()
+        if self.orc:
+            self.orc.initialize_observer()
 
     def features(self, reinit=None):
         if reinit is None:

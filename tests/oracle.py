@@ -25,11 +25,12 @@ OBSERVERS = {"": 0, "thresh2": 1, "localmax": 2, "nhood2": 3, "basic": 4, "basic
 _libs: dict[str, ctypes.CDLL] = {}
 
 
-def _lib(model: str) -> ctypes.CDLL:
-    if model not in _libs:
-        path = build_oracle.oracle_path(model)
-        if not path.exists():
-            path = build_oracle.build_model(model)
+def _lib(model: str, variant: str = "") -> ctypes.CDLL:
+    key = model + ("/" + variant if variant else "")
+    if key not in _libs:
+        path = build_oracle.oracle_path(model, variant)
+        if variant or not path.exists():
+            path = build_oracle.build_model(model, variant=variant)
         lib = ctypes.CDLL(str(path))
         vp, sz, dp, ip = ctypes.c_void_p, ctypes.c_size_t, ctypes.POINTER(ctypes.c_double), ctypes.c_int
         u64p = ctypes.POINTER(ctypes.c_uint64)
@@ -51,6 +52,8 @@ def _lib(model: str) -> ctypes.CDLL:
         lib.oracle_features.argtypes = [vp, ip]
         lib.oracle_last_steps.restype = ctypes.c_ulonglong
         lib.oracle_last_steps.argtypes = [vp]
+        lib.oracle_last_trials.restype = ctypes.c_ulonglong
+        lib.oracle_last_trials.argtypes = [vp]
         lib.oracle_last_warm_steps.restype = ctypes.c_ulonglong
         lib.oracle_last_warm_steps.argtypes = [vp]
         lib.oracle_size.restype = ctypes.c_longlong
@@ -60,8 +63,10 @@ def _lib(model: str) -> ctypes.CDLL:
         lib.oracle_xoroshiro_next.argtypes = [u64p, u64p, sz]
         lib.oracle_randn.argtypes = [ip, u64p, dp, sz]
         lib.oracle_cm_eval.argtypes = [ip, ip, dp, dp, dp, sz]
-        _libs[model] = lib
-    return _libs[model]
+        if variant == "count":
+            lib.oracle_counts.argtypes = [dp, sz]
+        _libs[key] = lib
+    return _libs[key]
 
 
 def _dptr(a: np.ndarray):
@@ -77,9 +82,11 @@ class OracleEnsemble:
 
     def __init__(self, model: str, n_var: int, n_par: int, n_aux: int = 0, n_noise: int = 0,
                  stepper: str = "rk4", observer: str = "", precision: str = "double",
-                 n_store: int = 0, kind: str = "features", threads: int | None = None):
-        self.lib = _lib(model)
-        prec = {"single": 1, "double": 2}[precision]
+                 n_store: int = 0, kind: str = "features", threads: int | None = None, variant: str = ""):
+        if precision == "count":  # FP64 semantics on the operation-counting type
+            variant = "count"
+        self.lib = _lib(model, variant)
+        prec = {"single": 1, "double": 2, "count": 9}[precision]
         kinds = {"base": 0, "features": 1, "trajectory": 2}
         self.h = self.lib.oracle_create(kinds[kind], prec, STEPPERS[stepper], OBSERVERS[observer],
                                         n_var, n_par, n_aux, n_noise, n_store)
@@ -160,6 +167,11 @@ class OracleEnsemble:
         return int(self.lib.oracle_last_steps(self.h))
 
     @property
+    def last_trials(self) -> int:
+        """Step trials of the last pass: its steps plus the rejected trials."""
+        return int(self.lib.oracle_last_trials(self.h))
+
+    @property
     def last_warm_steps(self) -> int:
         return int(self.lib.oracle_last_warm_steps(self.h))
 
@@ -169,6 +181,28 @@ class OracleEnsemble:
         out = np.zeros(max(n, 0), dtype=np.float64)
         assert self.lib.oracle_get(self.h, name.encode(), _dptr(out), out.size) == 0
         return out
+
+    COUNT_NAMES = ("add", "mul", "div", "fma", "exp", "log", "pow", "pown", "sqrt", "other")
+    PHASES = ("transient", "warmup", "features", "trajectory")
+    BUCKETS = ("rejected", "accepted", "failed", "all")
+
+    def counts(self) -> dict:
+        """Operation counts of an ensemble created with precision="count":
+        {phase: {"trials": {accepted, rejected, failed},
+                 bucket: {count name: n}}} (oracle/flop_count.hpp)."""
+        nc, npb = len(self.COUNT_NAMES), len(self.BUCKETS)
+        out = np.zeros(len(self.PHASES) * npb * nc + len(self.PHASES) * 3)
+        assert self.lib.oracle_counts(_dptr(out), out.size) == 0
+        res = {}
+        for p, ph in enumerate(self.PHASES):
+            d = {}
+            for b, bk in enumerate(self.BUCKETS):
+                base = (p * npb + b) * nc
+                d[bk] = {nm: int(out[base + c]) for c, nm in enumerate(self.COUNT_NAMES)}
+            t0 = len(self.PHASES) * npb * nc + 3 * p
+            d["trials"] = {"rejected": int(out[t0]), "accepted": int(out[t0 + 1]), "failed": int(out[t0 + 2])}
+            res[ph] = d
+        return res
 
     def get_rng(self) -> np.ndarray:
         n = self.lib.oracle_size(self.h, b"rng")

@@ -273,3 +273,61 @@ def test_large_ensemble_variant_bitexact(gpu_available, prec):
     parts = [run(x0[:, k * q:(k + 1) * q], p[:, k * q:(k + 1) * q]) for k in range(4)]
     assert np.array_equal(F, np.concatenate([Fk for Fk, _ in parts], axis=1))
     assert np.array_equal(xf.reshape(4, n), np.concatenate([xk.reshape(4, q) for _, xk in parts], axis=1))
+
+
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+def test_controller_long_retry_and_stall(stepper):
+    """The adaptive controller's two ends of the retry loop, on the device
+    against the oracle (FP64) bit for bit: x' = -lambda x with dt = 1 from
+    t = 0 and lambda ~ 1e31 needs more than 100 halvings before the first
+    step is accepted (members 0-2); from t0 = 1 - 13 2^-53 with lambda = 1e40
+    every trial fails and the trial at hmin rounds to a longer step, so it
+    would repeat forever (members 3-5): each step fails at hmin and leaves t
+    and x unchanged.  (Float modes overflow on these right-hand sides.)"""
+    sp = (1.0, 1.0, 1e-12, 1e-6, 3, 1000, 1)
+    lam = np.array([1e31, 2e31, 5e30])
+    for t0, lams in ((0.0, lam), (1.0 - 13 * 2.0 ** -53, np.full(3, 1e40))):
+        print(f"controller case {stepper} t0={t0!r} lambda={lams}", flush=True)
+        p = Pair("decay", stepper, "", "double", kind="base", sp=sp)
+        p.set_problem_data(np.ones((1, 3)), lams[None, :])
+        p.set_tspan(t0, 2.0)
+        p.transient()
+        p.assert_bitexact(("xf", "dt", "tf"))
+        tf = p.gpu_get("tf")
+        if t0 == 0.0:
+            assert np.all(tf > 0) and np.all(tf < 1e-29)
+        else:
+            assert np.all(tf == t0) and np.all(p.gpu_get("dt") == 2.0 ** -49)
+
+
+@pytest.mark.parametrize("observer", ["thresh2", "localmax", "nhood2"])
+def test_continuation_equals_one_long_run(observer):
+    """Observer continuation on the device, with the exact setup that pins it
+    in the oracle (tests/test_oracle_continuation.py): harmonic oscillator,
+    rk4, dt = 1/64, spans ending half a step past a multiple of dt.  The
+    continued features (initialise over the whole span, features over
+    (0, 17), shift_x0, features over (0, 23)) equal one features() over
+    (0, 40) bit for bit on the device, and both equal the oracle."""
+    dt = 1.0 / 64
+    sp = (dt, 1.0, 1e-6, 1e-3, 10 ** 7, 1000, 1)
+    obs = dict(max_event_count=12)
+    n = 70
+    x0 = np.vstack([np.ones(n), np.zeros(n)])
+    om = np.linspace(1.5, 3.1, n)[None, :]
+    whole = Pair("harmonic", "rk4", observer, "double", sp=sp, obs=obs)
+    whole.set_problem_data(x0, om)
+    whole.set_tspan(0.0, 40.0 - dt / 2)
+    whole.features(True)
+    whole.assert_bitexact(("F", "xf", "tf"))
+    split = Pair("harmonic", "rk4", observer, "double", sp=sp, obs=obs)
+    split.set_problem_data(x0, om)
+    split.set_tspan(0.0, 40.0 - dt / 2)
+    split.initialize_observer()
+    split.set_tspan(0.0, 17.0 - dt / 2)
+    split.features()
+    split.shift_x0()
+    split.set_tspan(0.0, 23.0 - dt / 2)
+    split.features()
+    split.assert_bitexact(("F", "xf", "tf"))
+    assert np.array_equal(split.gpu_get("F"), whole.gpu_get("F"))
+    assert np.array_equal(split.gpu_get("tf") + 17.0, whole.gpu_get("tf"))

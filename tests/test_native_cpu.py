@@ -102,10 +102,15 @@ def test_pybind_defaults_match_reference():
     pi.vars = ["u"]
     assert pi.num_var == 1
     for cls in ("SimulatorBase", "FeatureSimulatorBase", "TrajectorySimulatorBase", "OpenCLResource",
-                "CLDeviceType", "CLVendor", "DeviceInfo", "PlatformInfo", "LogLevel"):
+                "CLDeviceType", "CLVendor", "DeviceInfo", "PlatformInfo", "LogLevel", "LoggerSingleton"):
         assert hasattr(w, cls)
     for fn in ("query_opencl", "_print_opencl", "get_logger"):
         assert hasattr(w, fn)
+    # get_logger returns the LoggerSingleton instance (CLODEpython.cpp:87-94)
+    logger = w.get_logger()
+    assert type(logger).__name__ == "LoggerSingleton"
+    for fn in ("set_log_level", "set_log_pattern", "get_log_level"):
+        assert hasattr(logger, fn)
 
 
 @pytest.mark.parametrize("observer,nvar,naux,nstore", [
@@ -218,7 +223,40 @@ def test_code_object_cache_checksum(tmp_path):
     compile_in_fresh_process()
     rebuilt = path.read_bytes()
     assert rebuilt[:32] == data[:32] and rebuilt == data
+    # a truncated file, and a header whose size field claims more than the
+    # file holds (2^40 bytes: checked against the file length before any
+    # allocation), are ignored and rebuilt as well
+    path.write_bytes(data[: 32 + n // 3])
+    compile_in_fresh_process()
+    assert path.read_bytes() == data
+    huge = bytearray(data)
+    huge[16:24] = (1 << 40).to_bytes(8, "little")
+    path.write_bytes(bytes(huge))
+    compile_in_fresh_process()
+    assert path.read_bytes() == data
     assert not list(tmp_path.glob("*.tmp*"))
+
+
+def test_code_object_cache_key_covers_the_toolchain(tmp_path):
+    """The cache key includes the toolchain identity (HIP release, runtime
+    version, the loaded hipRTC library and CLODE_TOOLCHAIN_ID): the same
+    program under another identity text gets another cache file, and the
+    original file is left alone."""
+    src = os.path.join(MODELS, "van_der_pol.cl")
+
+    def compile_with(extra_env):
+        env = dict(os.environ)
+        env.pop("CLODE_TOOLCHAIN_ID", None)
+        env.update(extra_env)
+        subprocess.run(["python3", "-c", _COMPILE_INTO, src, str(tmp_path)], check=True, cwd=REPO, env=env)
+
+    compile_with({})
+    first = sorted(p.name for p in tmp_path.glob("clode_*.co"))
+    compile_with({})
+    assert sorted(p.name for p in tmp_path.glob("clode_*.co")) == first
+    compile_with({"CLODE_TOOLCHAIN_ID": "another compiler"})
+    both = sorted(p.name for p in tmp_path.glob("clode_*.co"))
+    assert len(first) == 1 and len(both) == 2 and first[0] in both
 
 
 def test_unknown_names_and_bad_input():

@@ -196,3 +196,71 @@ def test_nhood1_without_events_reports_wrapped_period_count():
     F = o.get("F")
     assert np.all(F[:6] == 0)
     assert F[6 + 5] == 2.0 ** 32 - 1  # eventcount - 1 as unsigned with no event
+
+
+def test_thresh2_harmonic_durations_and_active_dip():
+    """x = A cos(w t) with the default thresholds: the up event is the rising
+    crossing of -0.4 A (phase 2 pi - acos(-0.4)), the down transition the
+    falling crossing of -0.6 A (phase acos(-0.6)).  Up duration
+    (acos(-0.6) + acos(-0.4)) / w, down duration the rest of the period, duty
+    their ratio to the period.  activeDip = time mean of x over the down state
+    (phases acos(-0.6) .. 2 pi - acos(-0.4)) minus the last local minimum
+    (-A): A (sin th2 - sin th1) / (th2 - th1) + A.  Events are step points
+    (dt = 1e-3), so durations are exact to a step."""
+    omega, A = 2.0, 1.5
+    F = dict(zip(names_thresh2(["x", "y"], [], 0), harmonic_features("thresh2", omega, A)))
+    th1, th2 = math.acos(-0.6), 2 * math.pi - math.acos(-0.4)
+    up = (math.acos(-0.6) + math.acos(-0.4)) / omega
+    down = 2 * math.pi / omega - up
+    for stat in ("max", "min", "mean"):
+        assert F[f"{stat} upDuration"] == pytest.approx(up, abs=2.5e-3), stat
+        assert F[f"{stat} downDuration"] == pytest.approx(down, abs=2.5e-3), stat
+        assert F[f"{stat} duty"] == pytest.approx(up / (up + down), abs=2e-3), stat
+    dip = A * (math.sin(th2) - math.sin(th1)) / (th2 - th1) + A
+    assert dip == pytest.approx(0.26603, abs=1e-5)  # 0.17735 A
+    for stat in ("max", "min", "mean"):
+        assert F[f"{stat} activeDip"] == pytest.approx(dip, abs=5e-3), stat
+    # a mistaken dip (e.g. the down-state mean alone, or the mean over the
+    # whole period, 0) is off by at least 0.26
+    assert F["event count"] == 13  # up crossings at t = 2.150 + pi k <= 40
+
+
+def two_tone_features(observer, b=0.6, obs=None):
+    o = feature_sim("two_tone", 5, 1, 0, "rk4", observer, "double",
+                    (1.0 / 256, 1.0, 1e-6, 1e-3, 10 ** 7, 10 ** 7, 1),
+                    dict(dict(e_var_ix=4, f_var_ix=4), **(obs or {})))
+    o.set_problem_data(np.array([[1.0], [0.0], [1.0], [0.0], [1.0 + b]]), np.array([[b]]))
+    o.set_tspan((0.0, 20 * math.pi))
+    o.features()
+    return o.get("F")
+
+
+def test_thresh2_and_nhood2_count_two_maxima_per_period():
+    """s = cos t + 0.6 cos 2t: two local maxima per period (1.6 at t = 0,
+    -0.4 at t = pi).  The thresholds (0.3 / 0.2 of the range above the
+    minimum -0.808) put one up event per period, so thresh2 counts 2 peaks
+    per period; nhood2's centre is the first downward crossing of 0.2 of the
+    range and one exit from its ball happens per period, with 2 maxima in
+    between.  A rule that counted sign changes of ds differently (e.g. every
+    extremum, or only the largest maximum) gives 4 or 1."""
+    F = dict(zip(names_thresh2(["u1", "v1", "u2", "v2", "s"], [], 0), two_tone_features("thresh2")))
+    assert F["mean period"] == pytest.approx(2 * math.pi, abs=1e-2)
+    assert F["max peaks"] == 2 and F["min peaks"] == 2 and F["mean peaks"] == 2
+    assert F["max s"] == pytest.approx(1.6, abs=1e-6)  # sampled at steps of 1/256
+    smin = -1 / 2.4 + 0.6 * (2 * (1 / 2.4) ** 2 - 1)
+    assert F["min s"] == pytest.approx(smin, abs=1e-4)
+    Fn = two_tone_features("nhood2")
+    # nhood2: period max/min/mean, then peaks max/min/mean
+    assert Fn[2] == pytest.approx(2 * math.pi, abs=1e-2)
+    assert Fn[3] == 2 and Fn[4] == 2 and Fn[5] == 2
+
+
+def test_localmax_alternating_maxima():
+    """localmax on the same s: every maximum is an event, so inter-maximum
+    intervals are pi; amplitudes (maximum minus the last local minimum)
+    alternate between 1.6 - smin and -0.4 - smin."""
+    smin = -1 / 2.4 + 0.6 * (2 * (1 / 2.4) ** 2 - 1)
+    F = two_tone_features("localmax")
+    assert F[0] == pytest.approx(math.pi, abs=1e-2) and F[1] == pytest.approx(math.pi, abs=1e-2)
+    assert F[3] == pytest.approx(1.6 - smin, abs=1e-4)   # max amplitude
+    assert F[4] == pytest.approx(-0.4 - smin, abs=1e-4)  # min amplitude

@@ -105,3 +105,45 @@ def test_linear_drag_trajectory_five_digits(prec):
     for a, b in ((x[1:, 0], y1[1:]), (x[1:, 1], y2[1:])):
         np.testing.assert_allclose(a, b, rtol=5e-5, atol=1e-6 if prec == "single" else 0)
     assert n == 5001 or n == 5002
+
+
+def first_adaptive_step(stepper, lam, t0, t1, max_steps=1):
+    o = OracleEnsemble("decay", 1, 1, stepper=stepper, kind="base", threads=1)
+    o.set_solver_params(1.0, 1.0, 1e-12, 1e-6, max_steps, 1000, 1)
+    o.set_problem_data(np.ones((1, 1)), np.full((1, 1), lam))
+    o.set_tspan((t0, t1))
+    o.transient()
+    return o.get("xf")[0], o.get("tf")[0], o.get("dt")[0]
+
+
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+def test_controller_halves_past_one_hundred_trials(stepper):
+    """x' = -1e31 x from t = 0 with dt = 1: the first trial fails by ~1e31,
+    so the step shrinks to 0.1 (the maximal first shrink: 0.5 for bs23, 0.1
+    for dopri5) and then halves until lambda h is small; that takes more than
+    100 halvings.  The reference keeps halving until hmin (~8e-323 at t = 0,
+    adaptive_explicit_step.clh:44-50), so the step is accepted at
+    h = s 2^-k (exactly: t = 0 + h is exact) and x(h) = exp(-lambda h).  A
+    bound on trials of 100 would instead fail the step at hmin (t stays 0)."""
+    x, tf, dt = first_adaptive_step(stepper, 1e31, 0.0, 1.0)
+    shrink = 0.5 if stepper == "bs23" else 0.1
+    assert tf > 0.0
+    k = math.log2(shrink / tf)
+    assert k == round(k) and k > 100, k
+    assert tf == shrink * 2.0 ** -round(k)
+    assert x == pytest.approx(math.exp(-1e31 * tf), rel=1e-5)
+    assert 1e31 * tf < 1.0  # the accepted step resolves the decay
+
+
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+def test_controller_fails_a_stalled_trial_at_hmin(stepper):
+    """t0 = 1 - 13 2^-53, just below a power of two: hmin = 16 ulp = 2^-49,
+    but t0 + hmin rounds (to even) to 1 + 2^-51, a step of 17 ulp > hmin.  A
+    right-hand side whose error no step can meet (x' = -1e40 x) is rejected
+    down to that trial, which then repeats forever in the reference's loop
+    (its exit needs a step <= hmin).  Here it fails at hmin: each of the
+    max_steps = 3 steps leaves t and x unchanged with dt = hmin."""
+    t0 = 1.0 - 13 * 2.0 ** -53
+    assert (t0 + 2.0 ** -49) - t0 == 17 * 2.0 ** -53
+    x, tf, dt = first_adaptive_step(stepper, 1e40, t0, 2.0, max_steps=3)
+    assert tf == t0 and x == 1.0 and dt == 2.0 ** -49
