@@ -26,11 +26,18 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
 #include <vector>
 
 #include "../../../include/clode_mlp.h"
+
+// Staging memory of clode_mlp_screen_grid_noalloc: one byte per grid cell for
+// the dilation's second flag buffer.  Each device holds its own copy of the
+// array; host code reaches it only through the address that
+// hipGetSymbolAddress returns on that device (clode_mlp_create records it).
+static __device__ __align__(256) unsigned char scratch[CLODE_MLP_NOALLOC_MAX_CELLS];
 
 namespace {
 
@@ -785,6 +792,24 @@ void check(hipError_t e, const char *what)
         throw MlpError(std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// Device address of the static `scratch` array, per device: looked up once
+// per device, with that device current, by the first clode_mlp_create on it.
+std::mutex g_scratch_mu;
+std::vector<unsigned char *> g_scratch_addr;
+
+unsigned char *scratch_address(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if ((size_t)device >= g_scratch_addr.size())
+        g_scratch_addr.resize((size_t)device + 1, nullptr);
+    if (!g_scratch_addr[device]) {
+        void *p = nullptr;
+        check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+        g_scratch_addr[device] = static_cast<unsigned char *>(p);
+    }
+    return g_scratch_addr[device];
+}
+
 template <class T>
 struct Dev {
     T *p = nullptr;
@@ -835,6 +860,7 @@ struct clode_mlp {
     long long adamStep = 0;
     double lastMs = 0;
     size_t rows = 0;
+    unsigned char *stage = nullptr;  // this device's static scratch (no-allocation screening)
 
     void ensureRows(size_t n)
     {
@@ -1053,6 +1079,75 @@ T *need(T *p, const char *what)
         return CLODE_MLP_ERR_INVALID;     \
     }
 
+// ---- screening steps shared by the allocating and the no-allocation entry points
+
+void upload_target_scaling(clode_mlp *h)
+{
+    float tm[4] = {h->tmean[0], h->tmean[1], h->tstd[0], h->tstd[1]};
+    upload(h, h->dtm.p, tm, 16);
+}
+
+// the fused screening kernel over the nx x ny grid into device outputs
+void screen_fused(clode_mlp *h, int nx, int ny, float tol, float *P, float *A, int *C, uint8_t *F)
+{
+    const long long ncell = (long long)nx * ny;
+    ScreenArgs a;
+    std::memset(&a, 0, sizeof(a));
+    a.ncell = ncell;
+    a.nx = nx;
+    a.ny = ny;
+    a.Bff = h->dBff.p;
+    for (int l = 0; l < NLAYER; ++l) {
+        a.W[l] = h->pbf.p + h->L[l].woff;
+        a.bias[l] = h->p.p + h->L[l].boff;
+        if (((uintptr_t)a.W[l] & 15) != 0)  // the kernel reads 16-byte weight fragments
+            throw MlpError("screening weights are not 16-byte aligned");
+    }
+    a.tm = h->dtm.p;
+    a.tol = tol;
+    a.period = P;
+    a.amp = A;
+    a.cls = C;
+    a.flags = F;
+    const unsigned int nblk = (unsigned int)((ncell + SROWS - 1) / SROWS);
+    hipLaunchKernelGGL(mlp_screen_fused, dim3(nblk), dim3(256), 0, h->stream, a);
+    check(hipGetLastError(), "mlp_screen_fused launch");
+}
+
+// flags bit 1 (class boundary) into F, then `dilate` rounds of dilation
+// between F and F2 (two buffers of nx * ny bytes); returns the buffer that
+// holds the final flags
+const uint8_t *screen_flags(clode_mlp *h, int nx, int ny, int dilate, const int *C, uint8_t *F, uint8_t *F2)
+{
+    const long long ncell = (long long)nx * ny;
+    const unsigned int gb = (unsigned int)((ncell + 255) / 256);
+    hipLaunchKernelGGL(mlp_boundary, dim3(gb), dim3(256), 0, h->stream, C, nx, ny, F);
+    check(hipGetLastError(), "mlp_boundary launch");
+    for (int r = 0; r < dilate; ++r) {
+        hipLaunchKernelGGL(mlp_dilate, dim3(gb), dim3(256), 0, h->stream, F, nx, ny, F2);
+        check(hipGetLastError(), "mlp_dilate launch");
+        std::swap(F, F2);
+    }
+    return F;
+}
+
+// a caller's output pointer that must be device memory of the handle's device
+template <class T>
+void need_device(clode_mlp *h, T *p, const char *what)
+{
+    need(p, what);
+    hipPointerAttribute_t at;
+    std::memset(&at, 0, sizeof(at));
+    const hipError_t e = hipPointerGetAttributes(&at, p);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();  // leave no error behind for the next HIP call
+        throw std::invalid_argument(std::string(what) + " is not device memory");
+    }
+    if ((at.type != hipMemoryTypeDevice && at.type != hipMemoryTypeManaged) || at.device != h->device)
+        throw std::invalid_argument(std::string(what) + " is not device memory of device " +
+                                    std::to_string(h->device));
+}
+
 }  // namespace
 
 extern "C" {
@@ -1123,6 +1218,7 @@ int clode_mlp_create(clode_mlp **out, int device, unsigned long long seed)
         h->dBff.alloc(2 * NFF);
         h->dtm.alloc(4);
         h->loss.alloc(1);
+        h->stage = scratch_address(device);  // device is current (hipSetDevice above)
         upload(h, h->p.p, p.data(), p.size() * 4);
         upload(h, h->mask.p, mask.data(), mask.size());
         upload(h, h->dBff.p, h->Bff.data(), h->Bff.size() * 4);
@@ -1367,32 +1463,10 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
     dC.alloc(ncell);
     dF.alloc(ncell);
     dF2.alloc(ncell);
-    float tm[4] = {h->tmean[0], h->tmean[1], h->tstd[0], h->tstd[1]};
-    upload(h, h->dtm.p, tm, 16);
+    upload_target_scaling(h);
     check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
-    if (fused) {
-        ScreenArgs a;
-        std::memset(&a, 0, sizeof(a));
-        a.ncell = ncell;
-        a.nx = nx;
-        a.ny = ny;
-        a.Bff = h->dBff.p;
-        for (int l = 0; l < NLAYER; ++l) {
-            a.W[l] = h->pbf.p + h->L[l].woff;
-            a.bias[l] = h->p.p + h->L[l].boff;
-            if (((uintptr_t)a.W[l] & 15) != 0)  // the kernel reads 16-byte weight fragments
-                throw MlpError("screening weights are not 16-byte aligned");
-        }
-        a.tm = h->dtm.p;
-        a.tol = tol;
-        a.period = dP.p;
-        a.amp = dA.p;
-        a.cls = dC.p;
-        a.flags = dF.p;
-        const unsigned int nblk = (unsigned int)((ncell + SROWS - 1) / SROWS);
-        hipLaunchKernelGGL(mlp_screen_fused, dim3(nblk), dim3(256), 0, h->stream, a);
-        check(hipGetLastError(), "mlp_screen_fused launch");
-    }
+    if (fused)
+        screen_fused(h, nx, ny, tol, dP.p, dA.p, dC.p, dF.p);
     for (long long c0 = 0; !fused && c0 < ncell; c0 += chunk) {
         const int nb = (int)std::min<long long>(chunk, ncell - c0);
         hipLaunchKernelGGL(mlp_grid_inputs, dim3((nb + 255) / 256), dim3(256), 0, h->stream, c0, nb, nx, ny, h->X.p);
@@ -1403,19 +1477,46 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
                            h->dtm.p + 2, tol, dP.p + c0, dA.p + c0, dC.p + c0, dF.p + c0);
         check(hipGetLastError(), "mlp_predict_cells launch");
     }
-    const unsigned int gb = (unsigned int)((ncell + 255) / 256);
-    hipLaunchKernelGGL(mlp_boundary, dim3(gb), dim3(256), 0, h->stream, dC.p, nx, ny, dF.p);
-    check(hipGetLastError(), "mlp_boundary launch");
-    for (int r = 0; r < dilate; ++r) {
-        hipLaunchKernelGGL(mlp_dilate, dim3(gb), dim3(256), 0, h->stream, dF.p, nx, ny, dF2.p);
-        check(hipGetLastError(), "mlp_dilate launch");
-        std::swap(dF.p, dF2.p);
-    }
+    const uint8_t *res = screen_flags(h, nx, ny, dilate, dC.p, dF.p, dF2.p);
     check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     download(h, period, dP.p, ncell * 4);
     download(h, amp, dA.p, ncell * 4);
     download(h, cls, dC.p, ncell * 4);
-    download(h, flags, dF.p, ncell);
+    download(h, flags, res, ncell);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
+    h->lastMs = ms;
+    MLP_CATCH
+}
+
+int clode_mlp_screen_grid_noalloc(clode_mlp *h, int nx, int ny, float tol, int dilate, float *d_period, float *d_amp,
+                                  int *d_cls, unsigned char *d_flags)
+{
+    CHECK_H(h) MLP_TRY
+    if (nx < 1 || ny < 1)
+        throw std::invalid_argument("grid must have at least one cell");
+    const long long ncell = (long long)nx * ny;
+    if (ncell > CLODE_MLP_NOALLOC_MAX_CELLS)
+        throw std::invalid_argument("grid of " + std::to_string(ncell) + " cells exceeds the " +
+                                    std::to_string(CLODE_MLP_NOALLOC_MAX_CELLS) +
+                                    "-cell limit of the no-allocation screening (its static staging buffer)");
+    if (dilate < 0 || dilate > 64)
+        throw std::invalid_argument("dilation radius must be 0..64");
+    check(hipSetDevice(h->device), "hipSetDevice");
+    need_device(h, d_period, "period output");
+    need_device(h, d_amp, "amplitude output");
+    need_device(h, d_cls, "class output");
+    need_device(h, d_flags, "flags output");
+    if (!h->stage)
+        throw MlpError("the static scratch of this device has no address");
+    upload_target_scaling(h);
+    check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
+    screen_fused(h, nx, ny, tol, d_period, d_amp, d_cls, d_flags);
+    const uint8_t *res = screen_flags(h, nx, ny, dilate, d_cls, d_flags, h->stage);
+    if (res != d_flags)  // an odd number of dilation rounds ends in the staging buffer
+        check(hipMemcpyAsync(d_flags, res, (size_t)ncell, hipMemcpyDeviceToDevice, h->stream), "hipMemcpy D2D");
+    check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
+    check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     float ms = 0;
     (void)hipEventElapsedTime(&ms, h->ev0, h->ev1);
     h->lastMs = ms;

@@ -58,6 +58,9 @@ def _lib():
         lib.clode_mlp_screen_grid.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int, fp, fp,
                                               ip, ctypes.POINTER(ctypes.c_uint8)]
         lib.clode_mlp_last_ms.argtypes = [vp, ctypes.POINTER(ctypes.c_double)]
+        # outputs are device pointers (integers): nothing is allocated by the call
+        lib.clode_mlp_screen_grid_noalloc.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_float, ctypes.c_int,
+                                                      vp, vp, vp, vp]
         _LIB = lib
     return _LIB
 

@@ -69,6 +69,24 @@ int clode_mlp_screen_grid(clode_mlp *h, int nx, int ny, float tol, int dilate, f
 /* device time of the last train / screen call, milliseconds */
 int clode_mlp_last_ms(clode_mlp *h, double *ms);
 
+/* No counterpart in the reference (nor in the allocating call above): for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same screening
+ * as clode_mlp_screen_grid, with identical results, but the four outputs are
+ * caller-owned DEVICE buffers of nx * ny elements on the handle's device
+ * (d_period, d_amp: float; d_cls: int; d_flags: bytes), and the staging
+ * buffer of the dilation lives in a static device array of the library
+ * (CLODE_MLP_NOALLOC_MAX_CELLS bytes per device, held until the process
+ * ends).  Grids above CLODE_MLP_NOALLOC_MAX_CELLS cells are rejected with
+ * CLODE_MLP_ERR_INVALID.  The call allocates nothing and returns when the
+ * results are complete.  All calls on one device share the static array:
+ * calls that overlap in time on one device (several host threads, or several
+ * handles on one device used concurrently) race and must be serialised by
+ * the caller. */
+#define CLODE_MLP_NOALLOC_MAX_CELLS 4194304 /* 2048 x 2048 */
+int clode_mlp_screen_grid_noalloc(clode_mlp *h, int nx, int ny, float tol, int dilate, float *d_period, float *d_amp,
+                                  int *d_cls, unsigned char *d_flags);
+
 #ifdef __cplusplus
 }
 #endif
