@@ -62,13 +62,15 @@ def lactotroph_sim(precision, n_side=256):
 
 
 def one_step(sim):
-    """transient 5e3, then thresh2 warm-up + features over 2e4 (x0 <- xf)."""
+    """transient 5e3, then thresh2 warm-up + features over 2e4 (x0 <- xf),
+    and the feature matrix read back to the host, as the reference's
+    features() does (clode/features.py:467-507)."""
     sim.transient(t_span=(0.0, 5000.0))
     k_tr = sim.last_kernel_ms()
     sim.set_tspan((0.0, 20000.0))
     sim.initialize_observer()
     k_init = sim.last_kernel_ms()
-    sim.features(initialize_observer=False, fetch_results=False)
+    sim.features(initialize_observer=False, fetch_results=True)
     k_feat = sim.last_kernel_ms()
     return k_tr, k_init, k_feat
 
@@ -86,18 +88,25 @@ def run_lactotroph(precision, steps, warmup):
     t1 = time.perf_counter()
     member_steps = int(sim.get_step_counts().sum())
     wall = t1 - t0
-    # instrumented extra step: per-kernel member-steps for the roofline
-    counts = []
+    # instrumented extra step: per-kernel member-steps and step trials (steps
+    # plus rejected trials) for the roofline
+    counts, trials = [], []
+
+    def take():
+        s, t = int(sim.get_step_counts().sum()), int(sim.get_trial_counts().sum())
+        counts.append(s - sum(counts))
+        trials.append(t - sum(trials))
+
     sim.reset_step_counts()
     sim.transient(t_span=(0.0, 5000.0))
-    counts.append(int(sim.get_step_counts().sum()))
+    take()
     k_tr = sim.last_kernel_ms()
     sim.set_tspan((0.0, 20000.0))
     sim.initialize_observer()
-    counts.append(int(sim.get_step_counts().sum()) - counts[0])
+    take()
     k_init = sim.last_kernel_ms()
     sim.features(initialize_observer=False, fetch_results=False)
-    counts.append(int(sim.get_step_counts().sum()) - counts[0] - counts[1])
+    take()
     k_feat = sim.last_kernel_ms()
     out = sim.get_observer_results()
     return {
@@ -107,15 +116,55 @@ def run_lactotroph(precision, steps, warmup):
         "ms_per_step": 1e3 * wall / steps,
         "kernel_ms": {"transient": k_tr, "warmup": k_init, "features": k_feat},
         "kernel_member_steps": {"transient": counts[0], "warmup": counts[1], "features": counts[2]},
+        "kernel_trials": {"transient": trials[0], "warmup": trials[1], "features": trials[2]},
         "features": out.to_ndarray(),
         "feature_names": out.get_feature_names(),
     }
 
 
+def one_launch_sequence(precision, steps, r_three=None):
+    """The same step as one_step() with the transient, x0 <- xf, warm-up and
+    features of every member in one launch (clode_transient_features): wall
+    time per step with the feature readback, the kernel time, and whether the
+    features equal those of the three launches bit for bit."""
+    sim = lactotroph_sim(precision)
+    sim.transient_features((0.0, 5000.0), (0.0, 20000.0), fetch_results=False)  # program load, untimed
+    sim = lactotroph_sim(precision)
+    first = sim.transient_features((0.0, 5000.0), (0.0, 20000.0)).to_ndarray()
+    kernel_ms = [sim.last_kernel_ms()]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.transient_features((0.0, 5000.0), (0.0, 20000.0))
+        kernel_ms.append(sim.last_kernel_ms())
+    wall = time.perf_counter() - t0
+    ref = lactotroph_sim(precision)
+    one_step(ref)
+    same = bool(np.array_equal(first, ref.get_observer_results().to_ndarray()))
+    res = {"ms_per_step": 1e3 * wall / steps, "kernel_ms": float(np.median(kernel_ms[1:])),
+           "identical_features_to_three_launches": same}
+    if r_three is not None:
+        three = sum(r_three["kernel_ms"].values())
+        res["three_launch_kernel_ms"] = three
+        res["kernel_speedup"] = three / res["kernel_ms"]
+        res["wall_speedup"] = r_three["ms_per_step"] / res["ms_per_step"]
+    return res
+
+
+def kernel_tflops(r, phase, workload="lactotroph_256x256_dopri5_thresh2"):
+    """Algorithmic TFLOP/s of one launch of the timed sequence: the
+    generated operation table (clode_amd/perfmodel.py) priced with the
+    launch's device-counted steps and trials, over its HIP-event time."""
+    from clode_amd.perfmodel import launch_flops
+
+    fl = launch_flops(workload, phase, r["kernel_member_steps"][phase], r["kernel_trials"][phase])
+    return fl / (r["kernel_ms"][phase] / 1e3) / 1e12, fl
+
+
 def feature_error(f32c, f64, names):
-    """Relative error of fp32c features against FP64, for the period/peaks/V features."""
+    """Relative error of every fp32c feature against FP64 over the map's
+    members: median, p99, the fraction within 1e-4 and the fraction beyond 1 %."""
     res = {}
-    for key in ("mean period", "mean peaks", "max v", "min v", "mean v", "event count"):
+    for key in names:
         j = names.index(key)
         a, b = f32c[:, j], f64[:, j]
         scale = np.maximum(np.abs(b), 1e-30)
@@ -130,12 +179,32 @@ def feature_error(f32c, f64, names):
     return res
 
 
+def job_cores():
+    """The cores this process may use: its CPU affinity, capped by a cgroup
+    CPU quota when one is set (a shared GPU host gives each job a share of
+    its cores), with the host's core count beside them."""
+    host = os.cpu_count() or 1
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else host
+    quota = None
+    try:
+        q, period = open("/sys/fs/cgroup/cpu.max").read().split()[:2]
+        if q != "max":
+            quota = int(q) / int(period)
+    except (OSError, ValueError):
+        pass
+    if quota:
+        cores = max(1, min(cores, int(quota)))
+    return cores, host, quota
+
+
 def cpu_baseline(n_members=1024):
-    """CPU oracle on every (65536/n)-th member, same sequence, all host threads."""
+    """CPU oracle on every (65536/n)-th member, same sequence, every core
+    the job may use."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     from oracle import OracleEnsemble
 
-    threads = min(os.cpu_count() or 1, 16)
+    threads, host, quota = job_cores()
+    threads = min(threads, int(os.environ.get("CLODE_BENCH_CPU_THREADS", threads)))
     gk = np.linspace(0.0, 10.0, 256)
     gbk = np.linspace(0.0, 2.0, 256)
     idx = np.arange(0, 65536, 65536 // n_members)
@@ -156,9 +225,11 @@ def cpu_baseline(n_members=1024):
     steps += o.last_steps + o.last_warm_steps
     wall = time.perf_counter() - t0
     return {"value": steps / wall, "unit": "member-steps/s", "cores": threads, "kind": "port",
+            "host_cpus": host, "cpu_quota": quota,
             "sample": f"{n_members} of 65536 members (every {65536 // n_members}th), same transient+features "
                       f"sequence, FP64, {steps} member-steps in {wall:.1f} s on {threads} threads "
-                      f"({wall * threads:.0f} core-seconds); CPU: {_cpu_model()}"}
+                      f"({wall * threads:.0f} core-seconds; every core of the job's share: {threads} of the "
+                      f"host's {host}); CPU: {_cpu_model()}"}
 
 
 def _cpu_model():
@@ -200,7 +271,7 @@ def vdp_c1(n=4096, cpu_members=256):
     from oracle import OracleEnsemble
 
     idx = np.arange(0, n, n // cpu_members)
-    threads = min(os.cpu_count() or 1, 16)
+    threads = job_cores()[0]
     o = OracleEnsemble("van_der_pol", 2, 1, 0, 0, stepper="rk4", observer="thresh2", precision="double",
                        threads=threads)
     o.set_solver_params(0.01, 0.01, 1e-6, 1e-3, 10 ** 8, 1000000, 1)
@@ -393,6 +464,15 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
     t0 = time.perf_counter()
     block, slc = batch.autotune(stride=16, max_steps=4096)
     tune_s = time.perf_counter() - t0
+    rows, cached = batch.autotune_table()
+    tune = {"block": block, "slice": slc, "tune_s": tune_s, "from_cache": cached}
+    if len(rows):
+        order = np.argsort(rows[:, 2])
+        b0, b1 = rows[order[0]], rows[order[1]]
+        tune["trial_ms"] = {f"{int(r[0])}x{int(r[1])}": [round(float(v), 3) for v in r[2:]] for r in rows[order]}
+        # the winner's lead over the runner-up, against the spread of their trials
+        tune["lead_ms"] = float(b1[2] - b0[2])
+        tune["spread_ms"] = float(max(b0[4] - b0[3], b1[4] - b1[3]))
     for s in sims:
         s.reset_step_counts()
     t0 = time.perf_counter()
@@ -414,7 +494,7 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
     same = None
     if grid_rounds == rounds:
         same = all(np.array_equal(a, b.get_observer_results().to_ndarray()) for a, b in zip(F_graph, sims2))
-    return {"members": 4 * n_each, "rounds": rounds, "autotuned": {"block": block, "slice": slc, "tune_s": tune_s},
+    return {"members": 4 * n_each, "rounds": rounds, "autotuned": tune,
             "graph_sweep_s": graph_s, "s_per_round": graph_s / rounds, "member_steps": steps,
             "member_steps_per_s": steps / graph_s, "grid_rounds": grid_rounds, "grid_s_per_round": grid_s / grid_rounds,
             "speedup_vs_grid_per_round": (grid_s / grid_rounds) / (graph_s / rounds),
@@ -459,6 +539,19 @@ def surrogate_c5(precision, n=2048):
             float(m.train_loss[-1]) if m.train_loss is not None else None}
 
 
+def profiled_value(key, profile="features_lactotroph_256x256_dopri5_thresh2_fp64.md"):
+    """A figure measured by rocprofv3 and committed in a profile summary
+    (a line 'key value' under profiles/), or None."""
+    import re
+
+    try:
+        text = open(os.path.join(REPO, "profiles", profile)).read()
+    except OSError:
+        return None
+    m = re.search(r"^%s\s+([0-9.eE+-]+)\s*$" % re.escape(key), text, re.M)
+    return float(m.group(1)) if m else None
+
+
 def profiled_traffic(kernel="clode_features"):
     """Scaled HBM bytes per launch of the kernel, from the committed
     rocprofv3 FETCH_SIZE / WRITE_SIZE summary of this workload."""
@@ -481,8 +574,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-members", type=int, default=1024)
-    ap.add_argument("--extras", default="fp32c,c1,noise,c3,c4,c5",
-                    help="comma list of extra workloads: fp32c, c1, noise, c3, c4, c5 (or 'none'); --no-extra "
+    ap.add_argument("--extras", default="fp32c,seq,c1,noise,c3,c4,c5",
+                    help="comma list of extra workloads: fp32c, seq (one-launch sequence), c1, noise, c3, c4, c5 "
+                         "(or 'none'); --no-extra "
                          "drops them all")
     ap.add_argument("--c4-rounds", type=int, default=20)
     ap.add_argument("--mlp-kernels", action="store_true",
@@ -529,20 +623,34 @@ def main():
               f"(--gpus {args.gpus} requested)", file=sys.stderr)
         sys.exit(2)
 
-    from clode_amd.perfmodel import PEAK_TFLOPS, flops_per_step
+    from clode_amd.perfmodel import PEAK_TFLOPS, phase_counts
 
     r64 = run_lactotroph("double", args.steps, args.warmup)
     print(f"# fp64: {r64['member_steps_per_s']:.4e} member-steps/s, {r64['ms_per_step']:.1f} ms/step, "
           f"kernels {r64['kernel_ms']}", file=sys.stderr, flush=True)
-    fl = flops_per_step("lactotroph", "dopri5", "thresh2", 4, 1, "double")
-    feat_steps = r64["kernel_member_steps"]["features"]
-    feat_s = r64["kernel_ms"]["features"] / 1e3
-    achieved = feat_steps * fl["flops"] / feat_s / 1e12
+    achieved, feat_flops = kernel_tflops(r64, "features")
+    pc = phase_counts("lactotroph_256x256_dopri5_thresh2", "features")
+    executed = profiled_value("executed_fp64_flops_per_launch")
+    one_wave = profiled_value("fp64_fma_tflops_one_wave_per_simd", "valu_fma_rate.md")
     roofline = {"bound": "valu-fp64", "achieved": achieved, "peak": PEAK_TFLOPS["f64"], "unit": "TFLOP/s",
                 "frac": achieved / PEAK_TFLOPS["f64"], "traffic": profiled_traffic(),
                 "kernel": "clode_features (thresh2, dopri5, FP64)",
-                "flops_per_member_step": fl["flops"], "transcendentals_per_member_step": fl["transcendentals"],
-                "member_steps_per_launch": feat_steps, "launch_ms": r64["kernel_ms"]["features"]}
+                "algorithmic_flops_per_launch": feat_flops,
+                "flops_per_member_step": pc["step"], "flops_per_rejected_trial": pc["rejected_trial"],
+                "transcendentals_per_member_step": pc["transcendentals"],
+                "member_steps_per_launch": r64["kernel_member_steps"]["features"],
+                "trials_per_launch": r64["kernel_trials"]["features"], "launch_ms": r64["kernel_ms"]["features"],
+                "flop_table": "clode_amd/perfmodel.py (generated by oracle/gen_perfmodel.py)"}
+    if executed:
+        ex = executed / (r64["kernel_ms"]["features"] / 1e3) / 1e12
+        roofline["executed_tflops"] = ex
+        roofline["executed_frac"] = ex / PEAK_TFLOPS["f64"]
+    if one_wave:
+        # the launch has exactly one wave per SIMD (65,536 members, 1,024 SIMDs)
+        roofline["one_wave_per_simd_fma_ceiling_tflops"] = one_wave
+        roofline["frac_of_one_wave_ceiling"] = achieved / one_wave
+        if executed:
+            roofline["executed_frac_of_one_wave_ceiling"] = roofline["executed_tflops"] / one_wave
 
     extra = {}
     wanted = set() if args.no_extra or args.extras == "none" else set(args.extras.split(","))
@@ -559,14 +667,17 @@ def main():
         r32 = run_lactotroph("fp32c", args.steps, args.warmup)
         print(f"# fp32c: {r32['member_steps_per_s']:.4e} member-steps/s, {r32['ms_per_step']:.1f} ms/step",
               file=sys.stderr, flush=True)
-        fl32 = flops_per_step("lactotroph", "dopri5", "thresh2", 4, 1, "fp32c")
-        a32 = r32["kernel_member_steps"]["features"] * fl32["flops"] / (r32["kernel_ms"]["features"] / 1e3) / 1e12
+        a32, _ = kernel_tflops(r32, "features")
         extra["fp32c"] = {
             "member_steps_per_s": r32["member_steps_per_s"], "ms_per_step": r32["ms_per_step"],
             "kernel_ms": r32["kernel_ms"], "speedup_vs_fp64_wall": r64["ms_per_step"] / r32["ms_per_step"],
             "features_kernel_tflops": a32, "features_kernel_frac_of_fp32_peak": a32 / PEAK_TFLOPS["f32"],
             "feature_error_vs_fp64": feature_error(r32["features"], r64["features"], r64["feature_names"]),
         }
+    if "seq" in wanted:
+        run_extra("one_launch_sequence_fp64", lambda: one_launch_sequence("double", args.steps, r64))
+        if "fp32c" in wanted:
+            run_extra("one_launch_sequence_fp32c", lambda: one_launch_sequence("fp32c", args.steps, r32))
     if "c1" in wanted:
         run_extra("vdp_4096_rk4_fp64", vdp_c1)
     if "noise" in wanted:
@@ -584,8 +695,16 @@ def main():
         print(f"# cpu baseline: {cpu['value']:.4e} member-steps/s on {cpu['cores']} threads", file=sys.stderr,
               flush=True)
 
+    # the second half of the metric: the 2048^2 map's wall time, plain and
+    # screened, with the share of cells that disagree with the plain map
+    maps = {}
+    for prec in ("fp64", "fp32c"):
+        m = extra.get(f"surrogate_2048_{prec}")
+        if m and "error" not in m:
+            maps[prec] = {"plain_map_s": m["plain_map_s"], "screened_map_s": m["screened_total_s"],
+                          "speedup": m["speedup_total"], "cells_disagreeing_frac": m["cells_disagreeing_frac"]}
     line = {
-        "metric": "member-steps/s (lactotroph 256x256 feature map, dopri5, thresh2)",
+        "metric": "member-steps/s + 2048\u00b2 feature-map wall time at \u22641% feature error vs plain",
         "value": r64["member_steps_per_s"],
         "unit": "member-steps/s",
         "n_gpus": 1,
@@ -601,6 +720,7 @@ def main():
                                "transient 5e3 ms + thresh2 warm-up and features 2e4 ms",
                    "members": 65536, "member_steps_per_step": r64["member_steps"] // max(args.steps, 1),
                    "kernel_ms": r64["kernel_ms"]},
+        "map_2048": maps or None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "extra": extra,

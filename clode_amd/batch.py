@@ -12,6 +12,8 @@ from __future__ import annotations
 
 from typing import List, Optional, Sequence, Tuple
 
+import numpy as np
+
 from .cpp.clode_cpp_wrapper import BatchSimulatorBase
 from .features import FeatureSimulator
 from .solver import Simulator
@@ -44,8 +46,18 @@ class BatchSimulator:
     def autotune(self, stride: int = 16, max_steps: int = 4096) -> Tuple[int, int]:
         """Pick the fastest (block size, slice) by timing the transient launch
         over every stride-th queued member, each capped at max_steps steps
-        (0: no cap); member state is restored."""
+        (0: no cap), three times per shape in interleaved rounds, and keeping
+        the fastest median; member state is restored.  The choice is kept in
+        the code-object cache directory and reused for the same program,
+        device and ensemble sizes (CLODE_AUTOTUNE_FRESH=1 re-times)."""
         return tuple(self._batch.autotune(stride, max_steps))
+
+    def autotune_table(self) -> Tuple[np.ndarray, bool]:
+        """The last autotune's trials as rows (block, slice, median ms,
+        fastest ms, slowest ms), and whether the shape came from the cache
+        (then the table is empty)."""
+        rows, cached = self._batch.autotune_table()
+        return np.asarray(rows), bool(cached)
 
     def _invalidate(self, features: bool = False, x0: bool = False) -> None:
         for s in self._sims:

@@ -60,6 +60,13 @@ struct ClodeArgs {
     unsigned int q_pad;
 };
 
+// Arguments of the one-launch transient -> features sequence (clode_sequence):
+// the same buffers, the transient's span in tr and the features' span in fe.
+struct ClodeSeqArgs {
+    ClodeArgs tr;
+    ClodeArgs fe;
+};
+
 #ifdef __cplusplus
 static_assert(sizeof(void *) == 8, "64-bit targets only");
 #endif

@@ -501,6 +501,35 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS
 {
     run_queue<FeaturesLane>(a);
 }
+
+// transient -> x0 <- xf -> observer warm-up and initialisation -> features,
+// member by member in one launch: the same lanes as the four separate calls,
+// each phase reading what the member's previous phase stored, so the results
+// are those of transient(), shift_x0(), features(true) with the two spans.
+template <class Lane>
+CLODE_DEV void run_member(const ClodeArgs &a, const SolverParamsR &sp, unsigned int i)
+{
+    Lane L;
+    L.begin(a, i);
+    while (L.running(sp))
+        L.advance(a, sp);
+    L.finish(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_sequence(const ClodeSeqArgs s)
+{
+    const unsigned int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= s.tr.nPts)
+        return;
+    const SolverParamsR sp = make_solver_params(s.tr);
+    run_member<TransientLane>(s.tr, sp, i);
+    const size_t n = s.tr.nPts;
+#pragma unroll
+    for (int j = 0; j < N_VAR; ++j)  // shift_x0 for this member
+        ((xstore *)s.tr.x0)[j * n + i] = ((const xstore *)s.tr.xf)[j * n + i];
+    run_member<InitObsLane>(s.fe, sp, i);
+    run_member<FeaturesLane>(s.fe, sp, i);
+}
 #endif  // CLODE_OBSERVER
 #endif  // CLODE_BATCH
 

@@ -170,7 +170,7 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
 #endif
 
 #if CLODE_STEPPER == 4
-#define CLODE_ERR_ORDER RCONST(2.0)
+#define CLODE_ROOT_N 3  // 1 / (error order 2 + 1)
 #define CLODE_MAX_SHRINK RCONST(0.5)
 #define CLODE_MAX_GROW RCONST(5.0)
 CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
@@ -205,7 +205,7 @@ CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const rea
 #endif
 
 #if CLODE_STEPPER == 5
-#define CLODE_ERR_ORDER RCONST(4.0)
+#define CLODE_ROOT_N 5  // 1 / (error order 4 + 1)
 #define CLODE_MAX_SHRINK RCONST(0.1)
 #define CLODE_MAX_GROW RCONST(5.0)
 CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
@@ -274,7 +274,8 @@ CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const rea
 #if CLODE_ADAPTIVE_STEP
 // Error-per-unit-step controller (MATLAB ode23/ode45 style): elementwise
 // relative error against max(|x|, |xnew|, abstol/reltol), infinity norm,
-// first rejection shrinks by max(MAX_SHRINK, 0.8 (rtol/err)^(1/(p+1))),
+// first rejection shrinks by max(MAX_SHRINK, 0.8 (rtol/err)^(1/(p+1))) (the
+// root is cm::root<p+1>, within 2 ulp of the reference's pow),
 // later rejections halve, growth after an unrejected step is capped at
 // MAX_GROW, and the step is clipped to the end of the span and to
 // [hmin, dtmax] with hmin = 16 ulp(t).  One call makes one trial: it returns
@@ -286,7 +287,6 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
                             const SolverParamsR &sp, realtype &dt, const realtype t1,
                             realtype aux[], realtype wi[], RngData &rd, StepCtl &c)
 {
-    const realtype EXPON = RCONST(1.0) / (CLODE_ERR_ORDER + RCONST(1.0));
     realtype tNew, normErr, err[N_VAR], newxi[N_VAR], newk1[N_VAR];
     realtype threshold = sp.abstol / sp.reltol;
     realtype hmin = RCONST(16.0) * fabs(fabs(nextafter(ti, RCONST(1.1) * t1)) - ti);
@@ -314,8 +314,10 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
     normErr = ZERO;
     for (int j = 0; j < N_VAR; j++)
         normErr = fmax(fabs(err[j]), normErr);
-    // the factor both outcomes use, formed once for the whole wave
-    const realtype factor = RCONST(0.8) * pow(sp.reltol / normErr, EXPON);
+    // the factor both outcomes use, formed once for the whole wave:
+    // 0.8 (rtol/err)^(1/(p+1)) (the reference's pow(rtol/err, 1/(p+1)),
+    // here the division-free fixed root shared with the CPU oracle)
+    const realtype factor = RCONST(0.8) * cm::root<CLODE_ROOT_N>(sp.reltol / normErr);
     if (normErr > sp.reltol) {
         if (newDt <= hmin) {
             dt = hmin;

@@ -24,6 +24,7 @@
 #undef N_AUX1
 #undef N_WIENER1
 #undef CLODE_ERR_ORDER
+#undef CLODE_ROOT_N
 #undef CLODE_MAX_SHRINK
 #undef CLODE_MAX_GROW
 #undef CLODE_EXPON

@@ -24,6 +24,7 @@
 //       and fma-based products), with the sign of x for odd integer y; the
 //       float version rounds the double result once.
 //  pown: binary exponentiation (exact operation sequence).
+//  root<N>: x^(1/N), N = 3, 5, by division-free Newton iteration (below).
 //
 // CM_FN is defined by the including translation unit (host or device).
 #ifndef CLODE_CM_MATH_H
@@ -194,6 +195,70 @@ CM_FN double pown(double x, int n)
         m >>= 1;
     }
     return n < 0 ? 1.0 / r : r;
+}
+
+// x^(1/N) for N = 3 or 5: the adaptive controller's factor (rtol/err)^(1/(p+1))
+// for the error orders p = 2 (bs23) and 4 (dopri5), with the special cases of
+// pow(x, 1/N): NaN for NaN or x < 0, 0 for 0, inf for inf.  Branch-free and
+// division-free:
+//   x = M 2^(N q) with M = m 2^r, m in [1/2, 1), 0 <= r < N;
+//   w ~ M^(-1/N) from a cubic in m and the table 2^(-r/N), then two Newton
+//   steps w <- w + w (1 - M w^N) / N (quadratic convergence from 3e-4);
+//   y = M w^(N-1) and one Newton step on y^N = M, scaled by 2^q.
+// Within 2 ulp of the exact root (checked in the test-suite against
+// long-double references); it differs from pow(x, RN(1/N)) by the exponent's
+// own rounding, |log x| 2^-55 relative.
+template <int N>
+CM_FN double root(double x)
+{
+    static_assert(N == 3 || N == 5, "root<N>: N = 3 or 5");
+    const double inf = __builtin_huge_val();
+    double xs = (x > 0.0 && x < inf) ? x : 1.0;
+    int e;
+    double m = frexp(xs, &e);  // xs = m 2^e, m in [0.5, 1)
+    int q = e >= 0 ? e / N : -((N - 1 - e) / N);  // floor(e / N)
+    int r = e - N * q;                            // 0 .. N-1
+    double M = ldexp(m, r);
+    // w0 ~ m^(-1/N) on [1/2, 1) (least-squares cubic, within 3e-4 relative),
+    // times 2^(-r/N)
+    double w;
+    if (N == 3) {
+        w = fma(fma(fma(-0x1.f8703efde20eap-2, m, 0x1.93be6ca766c32p+0), m, -0x1.02f68e33ca1c6p+1), m,
+                0x1.f03c9f32afbc9p+0);
+        const double c1 = 0x1.965fea53d6e3dp-1, c2 = 0x1.428a2f98d728bp-1;  // 2^(-1/3), 2^(-2/3)
+        w *= (r == 0) ? 1.0 : ((r == 1) ? c1 : c2);
+    } else {
+        w = fma(fma(fma(-0x1.ee2841a748840p-3, m, 0x1.91ce75060d2cfp-1), m, -0x1.0d5cc4c4abae0p+0), m,
+                0x1.82339d4c4e107p+0);
+        const double c1 = 0x1.bdb8cdadbe120p-1, c2 = 0x1.8406003b2ae5cp-1, c3 = 0x1.51cb453b9536cp-1,
+                     c4 = 0x1.2611186bae674p-1;  // 2^(-1/5) .. 2^(-4/5)
+        w *= (r == 0) ? 1.0 : ((r == 1) ? c1 : ((r == 2) ? c2 : ((r == 3) ? c3 : c4)));
+    }
+    // two Newton steps for w = M^(-1/N): relative error 3e-4 -> 5e-7 -> 2e-12
+    const double rN = 1.0 / N;  // folded: RN(1/N)
+    for (int it = 0; it < 2; ++it) {
+        double w2 = w * w;
+        double wN = (N == 3) ? w2 * w : (w2 * w2) * w;
+        double t = fma(-M, wN, 1.0);
+        w = fma(w * t, rN, w);
+    }
+    double w2 = w * w;
+    double wN1 = (N == 3) ? w2 : w2 * w2;  // w^(N-1)
+    double y = M * wN1;
+    double y2 = y * y;
+    double yN = (N == 3) ? y2 * y : (y2 * y2) * y;
+    y = fma(-(yN - M) * wN1, rN, y);
+    y = ldexp(y, q);
+    y = (x == 0.0) ? 0.0 : y;
+    y = (x == inf) ? inf : y;
+    return (x < 0.0 || !(x == x)) ? __builtin_nan("") : y;
+}
+
+// float: the double root, rounded once
+template <int N>
+CM_FN float root(float x)
+{
+    return (float)root<N>((double)x);
 }
 
 // ----------------------------------------------------------------- float --

@@ -430,6 +430,12 @@ public:
         check(clode_is_observer_initialized(sim_, &v));
         return v != 0;
     }
+    void transient_features(std::vector<double> tr, std::vector<double> fe)
+    {
+        if (tr.size() != 2 || fe.size() != 2)
+            throw py::value_error("spans must have two elements");
+        check(clode_transient_features(sim_, tr.data(), fe.data()));
+    }
     void features_reinit(bool reinit)
     {
         py::gil_scoped_release nogil;
@@ -594,6 +600,15 @@ public:
             check(clode_batch_autotune(b_, stride, max_steps, &b, &s));
         }
         return py::make_tuple(b, s);
+    }
+    py::tuple autotune_table()
+    {
+        int n = 0, cached = 0;
+        check(clode_batch_autotune_table(b_, nullptr, 0, &n, &cached));
+        py::array_t<double> rows({(py::ssize_t)n, (py::ssize_t)5});
+        if (n > 0)
+            check(clode_batch_autotune_table(b_, rows.mutable_data(), n, &n, &cached));
+        return py::make_tuple(rows, cached != 0);
     }
     double last_kernel_ms()
     {
@@ -835,6 +850,8 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("is_observer_initialized", &FeatureSimulatorBasePy::is_observer_initialized)
         .def("features", &FeatureSimulatorBasePy::features_reinit)
         .def("features", &FeatureSimulatorBasePy::features)
+        .def("transient_features", &FeatureSimulatorBasePy::transient_features, py::arg("transient_tspan"),
+             py::arg("features_tspan"))
         .def("get_observer_params", &FeatureSimulatorBasePy::get_observer_params)
         .def("get_observer_name", &FeatureSimulatorBasePy::get_observer_name)
         .def("get_f", &FeatureSimulatorBasePy::get_f)
@@ -877,6 +894,7 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("shift_x0", &BatchSimulatorBasePy::shift_x0)
         .def("sweep", &BatchSimulatorBasePy::sweep, py::arg("rounds"), py::arg("graph") = true)
         .def("autotune", &BatchSimulatorBasePy::autotune, py::arg("stride") = 16, py::arg("max_steps") = 4096)
+        .def("autotune_table", &BatchSimulatorBasePy::autotune_table)
         .def("last_kernel_ms", &BatchSimulatorBasePy::last_kernel_ms)
         .def("__len__", &BatchSimulatorBasePy::size);
 }

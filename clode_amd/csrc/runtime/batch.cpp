@@ -1,8 +1,14 @@
 // Batched work-queue launches over several simulators (see batch.h).
 #include "batch.h"
 
+#include <sys/stat.h>
+#include <unistd.h>
+
 #include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <fstream>
 
 namespace clode {
 
@@ -299,11 +305,50 @@ void Batch::sweep(int rounds, bool graph)
         s->observerInitialized = true;
 }
 
+// where the autotuner keeps its choice: the first simulator's code-object
+// cache directory, keyed by the batched program, the device, the ensemble
+// sizes, the stride and the step cap ("" when no directory is set)
+std::string Batch::tunePath(int stride, unsigned int maxSteps)
+{
+    const std::string &dir = sims_[0]->root;
+    if (dir.empty())
+        return std::string();
+    hipDeviceProp_t prop;
+    std::string dev = hipGetDeviceProperties(&prop, device_) == hipSuccess ? prop.gcnArchName : "?";
+    std::string key = progKey_ + "\n" + dev + "\n" + std::to_string(stride) + " " + std::to_string(maxSteps);
+    for (Simulator *s : sims_)
+        key += " " + std::to_string(s->nPts);
+    uint64_t h = 1469598103934665603ull;
+    for (unsigned char c : key) {
+        h ^= c;
+        h *= 1099511628211ull;
+    }
+    char name[64];
+    std::snprintf(name, sizeof(name), "autotune_%016llx.txt", (unsigned long long)h);
+    return dir + (dir.back() == '/' ? "" : "/") + name;
+}
+
 void Batch::autotune(int stride, unsigned int maxSteps, int &blockOut, int &sliceOut)
 {
     if (stride < 1)
         throw InvalidArgument("autotune stride must be at least 1");
     prepare();
+    program();
+    tuneTable.clear();
+    tuneFromCache = false;
+    const std::string path = tunePath(stride, maxSteps);
+    const char *fresh = std::getenv("CLODE_AUTOTUNE_FRESH");
+    if (!path.empty() && !(fresh && std::strcmp(fresh, "1") == 0)) {
+        std::ifstream f(path);
+        int b = 0, sl = 0;
+        if (f >> b >> sl && (b == 64 || b == 128 || b == 256) && sl >= 1) {
+            block_ = blockOut = b;
+            slice_ = sliceOut = sl;
+            tuneFromCache = true;
+            lastMs = 0.0;
+            return;
+        }
+    }
     // per-member state the transient kernel overwrites
     struct Saved {
         DeviceBuffer xf, rng, dt, tf, steps;
@@ -341,14 +386,21 @@ void Batch::autotune(int stride, unsigned int maxSteps, int &blockOut, int &slic
     };
     const int blocks[] = {64, 128, 256};
     const int slices[] = {1, 4, 16, 64};
-    int bestB = block_, bestS = slice_;
-    double best = -1.0;
+    const int reps = 3;
+    std::vector<std::pair<int, int>> shapes;
+    for (int b : blocks)
+        for (int sl : slices)
+            shapes.push_back({b, sl});
+    std::vector<std::vector<double>> times(shapes.size());
+    const int keepB = block_, keepS = slice_;
     bool warmed = false;
     try {
-        for (int b : blocks)
-            for (int sl : slices) {
-                block_ = b;
-                slice_ = sl;
+        // interleaved rounds: slow drifts of the clock or of the machine's
+        // load touch every shape alike
+        for (int rep = 0; rep < reps; ++rep)
+            for (size_t c = 0; c < shapes.size(); ++c) {
+                block_ = shapes[c].first;
+                slice_ = shapes[c].second;
                 std::vector<ClodeArgs> a = args();
                 for (size_t k = 0; k < a.size(); ++k) {
                     a[k].q_stride = (unsigned int)stride;
@@ -365,26 +417,40 @@ void Batch::autotune(int stride, unsigned int maxSteps, int &blockOut, int &slic
                 enqueue("clode_batch_transient", a);
                 end();
                 restore();
-                if (best < 0 || lastMs < best) {
-                    best = lastMs;
-                    bestB = b;
-                    bestS = sl;
-                }
-                log_msg(LOG_DEBUG, "autotune block " + std::to_string(b) + " slice " + std::to_string(sl) + ": " +
-                                       std::to_string(lastMs) + " ms");
+                times[c].push_back(lastMs);
             }
     } catch (...) {
         hipStreamSynchronize(stream_);
-        block_ = bestB;
-        slice_ = bestS;
+        block_ = keepB;
+        slice_ = keepS;
         throw;
     }
     hip_check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-    block_ = bestB;
-    slice_ = bestS;
-    blockOut = bestB;
-    sliceOut = bestS;
-    lastMs = best;
+    size_t best = 0;
+    for (size_t c = 0; c < shapes.size(); ++c) {
+        std::vector<double> t = times[c];
+        std::sort(t.begin(), t.end());
+        tuneTable.push_back({shapes[c].first, shapes[c].second, t[t.size() / 2], t.front(), t.back()});
+        if (tuneTable[c].median < tuneTable[best].median)
+            best = c;
+        log_msg(LOG_DEBUG, "autotune block " + std::to_string(shapes[c].first) + " slice " +
+                               std::to_string(shapes[c].second) + ": median " + std::to_string(t[t.size() / 2]) +
+                               " ms");
+    }
+    block_ = blockOut = shapes[best].first;
+    slice_ = sliceOut = shapes[best].second;
+    lastMs = tuneTable[best].median;
+    if (!path.empty()) {
+        mkdir(sims_[0]->root.c_str(), 0755);
+        const std::string tmp = path + ".tmp" + std::to_string(getpid());
+        bool ok = false;
+        {
+            std::ofstream f(tmp);
+            ok = (bool)(f << block_ << " " << slice_ << "\n");
+        }
+        if (!ok || std::rename(tmp.c_str(), path.c_str()) != 0)
+            std::remove(tmp.c_str());
+    }
 }
 
 }  // namespace clode

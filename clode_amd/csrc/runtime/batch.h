@@ -41,9 +41,20 @@ public:
 
     // time the transient work-queue kernel on every stride-th queued member,
     // each trial capped at max_steps steps per member (0: no cap), for each
-    // candidate launch shape and keep the fastest (member state restored
-    // afterwards); returns the chosen shape
+    // candidate launch shape, three times in interleaved rounds, and keep the
+    // shape with the fastest median (member state restored afterwards);
+    // returns the chosen shape.  The choice is kept in the code-object cache
+    // directory (keyed by program, device, ensemble sizes, stride and cap) and
+    // reused by later runs without trials (CLODE_AUTOTUNE_FRESH=1 re-times).
     void autotune(int stride, unsigned int maxSteps, int &block, int &slice);
+    // the last autotune's trials: block, slice, median, min, max (ms); empty
+    // when the choice came from the cache
+    struct TuneRow {
+        int block, slice;
+        double median, min, max;
+    };
+    std::vector<TuneRow> tuneTable;
+    bool tuneFromCache = false;
 
     // compile the batched program for these simulators (no device needed)
     static void compileOnly(const std::vector<Simulator *> &sims);
@@ -72,6 +83,7 @@ private:
     void enqueueFeatures(int reinit, std::vector<bool> &initialised);
     void begin();
     void end();
+    std::string tunePath(int stride, unsigned int maxSteps);
 };
 
 }  // namespace clode

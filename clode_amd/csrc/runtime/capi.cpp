@@ -671,6 +671,25 @@ int clode_batch_autotune(clode_batch *batch, int stride, unsigned int max_steps,
         *slice = s;
     CLODE_CATCH
 }
+int clode_batch_autotune_table(clode_batch *batch, double *rows, int capacity, int *n_rows, int *from_cache)
+{
+    CHECK_BATCH(batch) CLODE_TRY
+    const auto &t = batch->b->tuneTable;
+    if (capacity < 0 || (capacity > 0 && !rows))
+        throw clode::InvalidArgument("autotune table needs capacity >= 0 and a row buffer");
+    const int n = std::min(capacity, (int)t.size());
+    for (int r = 0; r < n; ++r) {
+        rows[5 * r + 0] = t[r].block;
+        rows[5 * r + 1] = t[r].slice;
+        rows[5 * r + 2] = t[r].median;
+        rows[5 * r + 3] = t[r].min;
+        rows[5 * r + 4] = t[r].max;
+    }
+    *need(n_rows, "n_rows") = (int)t.size();
+    if (from_cache)
+        *from_cache = batch->b->tuneFromCache ? 1 : 0;
+    CLODE_CATCH
+}
 int clode_batch_last_ms(clode_batch *batch, double *ms)
 {
     CHECK_BATCH(batch) CLODE_TRY *need(ms, "ms") = batch->b->lastMs;
@@ -722,6 +741,14 @@ int clode_get_step_counts(clode_sim *sim, unsigned long long *out, size_t n)
     if (!v.empty())
         need(out, "output buffer");
     std::copy(v.begin(), v.end(), out);
+    CLODE_CATCH
+}
+int clode_transient_features(clode_sim *sim, const double *transient_tspan2, const double *features_tspan2)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    need(transient_tspan2, "transient span");
+    need(features_tspan2, "features span");
+    sim->s->transientFeatures(transient_tspan2[0], transient_tspan2[1], features_tspan2[0], features_tspan2[1]);
     CLODE_CATCH
 }
 int clode_get_trial_counts(clode_sim *sim, unsigned long long *out, size_t n)

@@ -714,6 +714,38 @@ void Simulator::features()
     launch("clode_features", makeArgs(), (unsigned int)nPts);
 }
 
+// transient(), shift_x0(), features(true) in one launch: each lane runs its
+// member through the three passes (clode_sequence).  Grid launch only; the
+// global spans only (per-member spans would give both passes the same span).
+void Simulator::transientFeatures(double tr0, double tr1, double fe0, double fe1)
+{
+    requireData("transient_features");
+    if (kind != 1)
+        throw InvalidArgument("transient_features needs a feature simulator");
+    if (memberTspan)
+        throw InvalidArgument("transient_features runs on the global spans; clear the per-member spans first");
+    resizeFeaturesVariables();
+    ClodeSeqArgs s;
+    setTspan({tr0, tr1});
+    s.tr = makeArgs();
+    setTspan({fe0, fe1});
+    s.fe = makeArgs();
+    ensureStream();
+    const unsigned int n = (unsigned int)nPts;
+    const unsigned int block = blockSize > 0 ? (unsigned int)blockSize : 64u;
+    hipFunction_t fn = gridFunction("clode_sequence", n, block);
+    void *params[] = {&s};
+    hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
+    hip_check(hipModuleLaunchKernel(fn, (n + block - 1) / block, 1, 1, block, 1, 1, 0, stream, params, nullptr),
+              "launch clode_sequence");
+    hip_check(hipEventRecord(ev1, stream), "hipEventRecord");
+    hip_check(hipStreamSynchronize(stream), "kernel clode_sequence");
+    float ms = 0.f;
+    hipEventElapsedTime(&ms, ev0, ev1);
+    lastKernelMs = ms;
+    observerInitialized = true;
+}
+
 // CLODEfeatures::getF (CLODEfeatures.cpp:261-275)
 std::vector<double> Simulator::getF()
 {

@@ -57,6 +57,7 @@ public:
     void initializeObserver();
     void features();
     void features(bool reinit);
+    void transientFeatures(double tr0, double tr1, double fe0, double fe1);
     std::vector<double> getF();
 
     // trajectory

@@ -235,6 +235,23 @@ class FeatureSimulator(Simulator):
             return self.get_observer_results()
         return None
 
+    def transient_features(self, t_transient: Tuple[float, float], t_features: Tuple[float, float],
+                           update_x0: bool = True, fetch_results: bool = True) -> Optional[ObserverOutput]:
+        """transient(t_transient), then features(t_features, initialize_observer=True)
+        in one kernel launch: every member runs its transient, warm-up and
+        features passes back to back (bit-identical to the separate calls).
+        Global spans only."""
+        self._integrator.transient_features(list(t_transient), list(t_features))
+        self._t_span = tuple(t_features)
+        self._device_features = None
+        self._device_final_state = self._device_dt = self._device_tf = None
+        self._device_initial_state = None
+        if update_x0:
+            self._integrator.shift_x0()
+        if fetch_results:
+            return self.get_observer_results()
+        return None
+
     def get_observer_results(self) -> ObserverOutput:
         if self._device_features is None:
             self._num_features = self._integrator.get_n_features()

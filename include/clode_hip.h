@@ -168,6 +168,16 @@ int clode_get_f(clode_sim *sim, double *out, size_t n);
 int clode_get_feature_names(clode_sim *sim, char *buf, size_t cap, size_t *needed);
 int clode_get_available_observers(clode_sim *sim, char *buf, size_t cap, size_t *needed);
 
+/* transient over transient_tspan2, x0 <- xf, and features over
+ * features_tspan2 with the observer re-initialised, in one kernel launch
+ * (each member runs its three passes back to back).  No reference
+ * counterpart: the reference's callers make the calls transient (CLODE.cpp:
+ * 468-493), shiftX0 (:502-515) and features(true) (CLODEfeatures.cpp:215-259),
+ * which this reproduces bit for bit.  The simulator's tspan is left at
+ * features_tspan2.  Global spans only (CLODE_ERR_INVALID with per-member
+ * spans set); always a grid launch. */
+int clode_transient_features(clode_sim *sim, const double *transient_tspan2, const double *features_tspan2);
+
 /* ---- trajectory (CLODEtrajectory.hpp:36-50) */
 int clode_trajectory(clode_sim *sim);
 int clode_get_trajectory_sizes(clode_sim *sim, size_t *nt, size_t *nx, size_t *naux);
@@ -251,9 +261,18 @@ int clode_batch_sweep(clode_batch *batch, int rounds, int use_graph);
 /* Time the transient launch over every stride-th queued member, each member
  * capped at max_steps steps (0: no cap; a cap keeps a heavy tail of long
  * members from dominating the trials), for each block size in {64, 128, 256}
- * and slice in {1, 4, 16, 64}, restoring the member state after each trial,
- * and keep the fastest. */
+ * and slice in {1, 4, 16, 64}, three times in interleaved rounds, restoring
+ * the member state after each trial, and keep the shape with the fastest
+ * median.  The choice is stored in the first simulator's code-object cache
+ * directory, keyed by the batched program, the device, the ensemble sizes,
+ * stride and max_steps; a later call with the same key takes it from there
+ * without trials unless the environment sets CLODE_AUTOTUNE_FRESH=1. */
 int clode_batch_autotune(clode_batch *batch, int stride, unsigned int max_steps, int *block_size, int *slice);
+/* The last autotune's trials, one row of 5 doubles per launch shape: block
+ * size, slice, median, fastest and slowest device time (ms).  Writes at most
+ * capacity rows, their number to *n_rows (0 when the shape came from the
+ * cache: *from_cache is then 1). */
+int clode_batch_autotune_table(clode_batch *batch, double *rows, int capacity, int *n_rows, int *from_cache);
 /* device time of the last batch operation, milliseconds */
 int clode_batch_last_ms(clode_batch *batch, double *ms);
 

@@ -322,10 +322,11 @@ struct Integrator {
         CLODE_ORACLE_TRIAL_BEGIN();
         const int N = pb.nVar;
         const bool bs = stepper == ST_BS23;
-        const R order = bs ? OC(2.0) : OC(4.0);
         const R maxShrink = bs ? OC(0.5) : OC(0.1);
         const R maxGrow = OC(5.0);
-        const R expon = OC(1.0) / (order + OC(1.0));
+        // the reference's pow(rtol/err, 1/(p+1)) as the fixed root x^(1/(p+1)),
+        // p = 2 (bs23) or 4 (dopri5), shared with the device (cm_math.h)
+        auto root = [bs](R v) { return bs ? cm::root<3>(v) : cm::root<5>(v); };
         R tNew, normErr;
         Buf<R> err, newxi, newk1;
         R newDt = *dt;
@@ -370,7 +371,7 @@ struct Integrator {
                 }
                 if (noFailedSteps) {
                     noFailedSteps = false;
-                    newDt *= fmax(maxShrink, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
+                    newDt *= fmax(maxShrink, OC(0.8) * root(sp.reltol / normErr));
                 } else {
                     newDt *= OC(0.5);
                 }
@@ -380,7 +381,7 @@ struct Integrator {
             }
         }
         if (noFailedSteps)
-            newDt *= fmin(maxGrow, OC(0.8) * cm::pow(sp.reltol / normErr, expon));
+            newDt *= fmin(maxGrow, OC(0.8) * root(sp.reltol / normErr));
         newDt = fmin(newDt, t1 - tNew);
         newDt = fmin(fmax(newDt, hmin), sp.dtmax);
         *dt = newDt;

@@ -177,6 +177,12 @@ inline fc::Counted pown(fc::Counted a, int n)
     fc::count(fc::C_POWN, a.var);
     return fc::Counted::make(cm::pown(a.v, n), a.var);
 }
+template <int N>
+inline fc::Counted root(fc::Counted a)  // counted with pow: the controller's (rtol/err)^(1/(p+1))
+{
+    fc::count(fc::C_POW, a.var);
+    return fc::Counted::make(cm::root<N>(a.v), a.var);
+}
 }  // namespace cm
 
 #define CLODE_ORACLE_TRIAL_BEGIN() fc::trial_begin()

@@ -10,7 +10,9 @@ against the CPU oracle and against the plain grid launch.
 * fp32c (no bit-exact oracle): batch results equal the same simulators run
   one by one with grid launches, and a graph-captured sweep equals the eager
   sweep;
-* the autotuner leaves every member's state as it found it.
+* the autotuner leaves every member's state as it found it, times every
+  launch shape three times and keeps the fastest median, and a later batch of
+  the same program and sizes takes the stored choice without trials.
 """
 import numpy as np
 import pytest
@@ -168,16 +170,33 @@ def test_graph_sweep_twice_equals_eager_four_rounds(gpu_available):
         assert np.array_equal(a, r)
 
 
-def test_autotune_restores_state(gpu_available):
+def test_autotune_restores_state(gpu_available, monkeypatch):
     pairs = [make_system(nm, "double", n, k) for k, (nm, n) in enumerate(zip(("lorenz", "ou"), (300, 400)))]
     b = batch_of(pairs)
+    monkeypatch.setenv("CLODE_AUTOTUNE_FRESH", "1")
     block, slice_ = b.autotune(4)
     assert block in (64, 128, 256) and slice_ in (1, 4, 16, 64)
     assert tuple(b.get_launch()) == (block, slice_)
+    rows, cached = b.autotune_table()
+    rows = np.asarray(rows)
+    assert not cached and rows.shape == (12, 5)
+    assert {(int(r[0]), int(r[1])) for r in rows} == {(bb, ss) for bb in (64, 128, 256) for ss in (1, 4, 16, 64)}
+    assert np.all(rows[:, 3] <= rows[:, 2]) and np.all(rows[:, 2] <= rows[:, 4]) and np.all(rows[:, 3] > 0)
+    best = rows[np.argmin(rows[:, 2])]
+    assert (int(best[0]), int(best[1])) == (block, slice_)
     b.transient()
     for p in pairs:
         p.orc.transient()
         p.assert_bitexact(("xf", "dt", "tf"))
+    # the choice persists: the same program and sizes take it without trials
+    monkeypatch.delenv("CLODE_AUTOTUNE_FRESH")
+    b2 = batch_of([make_system(nm, "double", n, k, oracle=False)
+                   for k, (nm, n) in enumerate(zip(("lorenz", "ou"), (300, 400)))])
+    b2.set_launch(256, 64 if (block, slice_) != (256, 64) else 1)
+    assert tuple(b2.autotune(4)) == (block, slice_)
+    rows2, cached2 = b2.autotune_table()
+    assert cached2 and np.asarray(rows2).shape == (0, 5)
+    assert tuple(b2.get_launch()) == (block, slice_)
 
 
 def test_batch_validation(gpu_available):

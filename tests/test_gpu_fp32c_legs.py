@@ -1,0 +1,276 @@
+"""The fp32-compensated legs the benchmark reports, each against the FP64
+CPU oracle, plus the full-size Van der Pol configuration bit for bit.
+
+fp32c and FP64 take different step sequences at the same tolerance (the
+float stage algebra perturbs the error estimates), so their difference is of
+the size of the method's own truncation error.  The yardstick for that error
+is FP64 at rtol against FP64 at rtol / 100 on the same members.  Stated bounds:
+
+* mixed batch (the four systems of the 2^20 mixed sweep, 256 members each,
+  two graph-captured rounds of transient + features in fp32c): for every
+  feature of every system and for xf, fp32c deviates from FP64 no more than
+  FP64 deviates from its own rtol / 100 solution, up to a factor of 2: the
+  median and the 90th percentile (over members) of |fp32c - fp64| are at most
+  2x those of |fp64 - fp64(rtol/100)|, plus 1e-4 of the feature's median
+  magnitude.  Event counts differ between fp32c and FP64 for at most 2 % of
+  the members, or 1.5x the share of members whose FP64 count changes at
+  rtol / 100 when that is larger: Lorenz's local maxima on a damped spiral
+  are ill-conditioned (FP64 at rtol and at rtol / 100 disagree on about a
+  third of those members, at rtol / 100 and rtol / 10^4 still on a sixth).
+  (Member by member the comparison is noisier: extrema over step-point
+  event times, such as the longest period, move with the step grid.)  Ornstein-Uhlenbeck
+  (Euler-Maruyama, no tolerance): the noise words equal the FP64 run's for
+  >= 99 % of the members and, on those, max / min / slopes agree to 1e-5
+  relative, xf to 1e-5 (1 + |x|) and the time mean to 1e-5 of the range.
+  Lorenz uses rho in [0, 13) (no chaos) and short spans (median
+  5), before the damped oscillation sinks below float resolution, where
+  fp32c's localmax counts rounding noise as maxima (documented limitation).
+* Lorenz 2^20, bs23 and dopri5: see tests/test_gpu_fullsize.py.
+* trajectory of 65,536 Lorenz members x 1024 samples: xf, dt and tf equal
+  those of an fp32c transient limited to the same 1024 steps, bit for bit;
+  on every 256th member (rho <= 15.6) each stored sample equals the FP64
+  oracle's trajectory, interpolated to its time by cubic Hermite, within
+  1e-4 (1 + |x|) for rho < 13 and with a median (over members) of the
+  largest error below 1e-5.
+* Euler-Maruyama: fp32c draws its normals in float exactly as single
+  precision does, so with a step size whose float time sums are exact the
+  noise words after a run equal the single-precision oracle's bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from helpers import MODEL_INFO, Pair, wrapper
+from oracle import OracleEnsemble
+
+pytestmark = pytest.mark.gpu
+
+MODELS = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "clode_amd", "models")
+ADAPT = (0.01, 1.0, 1e-8, 1e-6, 10 ** 9, 10, 1)
+NOISE = (0.05, 0.05, 1e-8, 1e-6, 10 ** 9, 10, 1)
+MIX_RATIO = 2.0
+
+
+def fp64_oracle(model, stepper, observer, sp, x0, pars, spans, seed, rtol_scale=1.0, kind="features"):
+    v, p, a, nn = MODEL_INFO[model]
+    o = OracleEnsemble(model, len(v), len(p), len(a), nn, stepper=stepper, observer=observer, kind=kind)
+    o.set_solver_params(sp[0], sp[1], sp[2] * rtol_scale, sp[3] * rtol_scale, *sp[4:])
+    o.set_observer_params(x_up_threshold=0.3, x_down_threshold=0.2)
+    o.set_problem_data(x0, pars)
+    if spans is not None:
+        o.set_member_tspan(spans)
+    o.seed_rng(seed)
+    return o
+
+
+def mixed_systems(n, seed=11):
+    rng = np.random.default_rng(seed)
+
+    def spans(median, sigma):
+        return np.vstack([np.zeros(n), median * np.exp(sigma * rng.standard_normal(n))])
+
+    return [
+        ("van_der_pol", "dopri5", "thresh2", ADAPT, np.vstack([2.0 * np.ones(n), np.zeros(n)]),
+         rng.uniform(0.1, 5.0, n)[None, :], spans(60.0, 0.5)),
+        ("lorenz", "dopri5", "localmax", ADAPT, rng.normal(1.0, 0.1, (3, n)),
+         np.vstack([10.0 * np.ones(n), rng.uniform(0.0, 13.0, n), np.full(n, 8.0 / 3.0)]), spans(5.0, 0.3)),
+        ("chay_keizer", "dopri5", "thresh2", ADAPT, np.tile([[-50.0], [0.01], [0.12]], (1, n)),
+         np.vstack([rng.uniform(550, 1050, n), 750.0 * np.ones(n), rng.uniform(0.095, 0.155, n)]),
+         spans(300.0, 0.5)),
+        ("ornstein_uhlenbeck", "seuler", "basicall", NOISE, np.zeros((1, n)),
+         np.vstack([np.ones(n), np.zeros(n), 0.5 * np.ones(n)]), spans(60.0, 1.0)),
+    ]
+
+
+def test_mixed_batch_fp32c_against_fp64(gpu_available):
+    n = 256
+    systems = mixed_systems(n)
+    pairs = []
+    for k, (model, stepper, observer, sp, x0, pars, spans) in enumerate(systems):
+        p = Pair(model, stepper, observer, "fp32c", sp=sp, oracle=False)
+        p.set_problem_data(x0, pars)
+        p.set_tspan(0.0, 1.0)
+        p.set_member_tspan(spans)
+        p.seed(100 + k)
+        pairs.append(p)
+    b = wrapper().BatchSimulatorBase([p.gpu for p in pairs])
+    b.set_launch(64, 16)
+    b.sweep(2, True)
+    report = {}
+    for k, ((model, stepper, observer, sp, x0, pars, spans), p) in enumerate(zip(systems, pairs)):
+        print(f"mixed fp32c vs fp64: {model}", flush=True)
+        refs = []
+        for scale in ((1.0, 1e-2) if stepper != "seuler" else (1.0,)):
+            o = fp64_oracle(model, stepper, observer, sp, x0, pars, spans, 100 + k, scale)
+            for _ in range(2):
+                o.transient()
+                o.shift_x0()
+                o.features()
+                o.shift_x0()
+            refs.append((o.get("F").reshape(-1, n), o.get("x0").reshape(-1, n), o.get_rng()))
+        A = np.array(p.gpu.get_f()).reshape(-1, n)
+        X = np.array(p.gpu.get_x0()).reshape(-1, n)
+        names = list(p.gpu.get_feature_names())
+        B, XB, RB = refs[0]
+        if stepper == "seuler":
+            same = np.all(np.array(p.gpu.get_rng_state()).reshape(2, n) == RB.reshape(2, n), axis=0)
+            assert same.mean() >= 0.99, same.mean()
+            assert np.all(np.abs(X - XB)[:, same] <= 1e-5 * (1 + np.abs(XB[:, same])))
+            for j, nm in enumerate(names):
+                a, r = A[j][same], B[j][same]
+                if nm == "mean x":
+                    span = B[names.index("max x")][same] - B[names.index("min x")][same]
+                    assert np.all(np.abs(a - r) <= 1e-5 * span), nm
+                else:
+                    assert np.all(np.abs(a - r) <= 1e-5 * np.abs(r) + 1e-12), nm
+            continue
+        Bf, XBf, _ = refs[1]
+        ec = names.index("event count")
+        miss, miss_ref = float(np.mean(A[ec] != B[ec])), float(np.mean(B[ec] != Bf[ec]))
+        assert miss <= max(0.02, 1.5 * miss_ref), (model, miss, miss_ref)
+        excess = {}
+        rows = [(nm, A[j], B[j], Bf[j]) for j, nm in enumerate(names)]
+        rows += [(f"xf[{j}]", X[j], XB[j], XBf[j]) for j in range(X.shape[0])]
+        for nm, a, r, rf in rows:
+            d, t = np.abs(a - r), np.abs(r - rf)
+            slack = 1e-4 * np.median(np.abs(r)) + 1e-12
+            for q in (50, 90):
+                bound = MIX_RATIO * np.percentile(t, q) + slack
+                excess[f"{nm} p{q}"] = float(np.percentile(d, q) / bound)
+        report[model] = excess
+        worst = max(excess, key=excess.get)
+        assert excess[worst] <= 1.0, (model, worst, excess[worst])
+
+
+@pytest.mark.slow
+def test_lorenz_trajectory_65536x1024_fp32c(gpu_available):
+    n, store, N2 = 65536, 1024, 1 << 20
+    x0 = np.random.default_rng(20240601).normal(1.0, 0.1, size=(N2, 3))[:n].T
+    rho = np.linspace(0.0, 250.0, N2)[:n]
+    pars = np.vstack([10.0 * np.ones(n), rho, np.full(n, 8.0 / 3.0)])
+    sp = (0.01, 1.0, 1e-8, 1e-6, 10 ** 8, store, 1)
+    print("trajectory 65536 x 1024 fp32c", flush=True)
+    p = Pair("lorenz", "dopri5", "", "fp32c", kind="trajectory", sp=sp, oracle=False)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 500.0)
+    p.gpu.trajectory()
+    ns = np.array(p.gpu.get_n_stored())
+    # members with rho near 0 decay and take steps of dtmax, reaching t = 500
+    # before their buffer is full; all others stop when it is full
+    assert np.all(ns <= store) and np.mean(ns == store) > 0.9
+    # the trajectory kernel runs the transient's step sequence: 1024 steps
+    b = Pair("lorenz", "dopri5", "", "fp32c", kind="base", sp=(0.01, 1.0, 1e-8, 1e-6, store, 10, 1), oracle=False)
+    b.set_problem_data(x0, pars)
+    b.set_tspan(0.0, 500.0)
+    b.gpu.transient()
+    for name in ("get_xf", "get_dt", "get_tf"):
+        assert np.array_equal(np.array(getattr(p.gpu, name)()), np.array(getattr(b.gpu, name)())), name
+    T = np.array(p.gpu.get_t()).reshape(store, n)
+    X = np.array(p.gpu.get_x()).reshape(store, 3, n)
+    idx = np.arange(0, n, 256)
+    o = fp64_oracle("lorenz", "dopri5", "", sp, x0[:, idx], pars[:, idx], None, 0, kind="trajectory")
+    o.set_tspan((0.0, 500.0))
+    o.trajectory()
+    m = idx.size
+    T64, X64, D64 = (o.get(k).reshape(store, -1, m) for k in ("t", "x", "dx"))
+    ns64 = o.get("nStored").astype(int)
+    worst = np.zeros(m)
+    for c, i in enumerate(idx):
+        t = T[: min(ns[i], store), i]
+        ok = t <= T64[min(ns64[c], store) - 1, 0, c]  # within the FP64 samples' time range
+        tt = T64[: min(ns64[c], store), 0, c]
+        kk = np.clip(np.searchsorted(tt, t[ok]) - 1, 0, tt.size - 2)
+        h = tt[kk + 1] - tt[kk]
+        s = (t[ok] - tt[kk]) / h
+        for j in range(3):
+            x, d = X64[:, j, c], D64[:, j, c]
+            xi = ((2 * s ** 3 - 3 * s ** 2 + 1) * x[kk] + (s ** 3 - 2 * s ** 2 + s) * h * d[kk] +
+                  (-2 * s ** 3 + 3 * s ** 2) * x[kk + 1] + (s ** 3 - s ** 2) * h * d[kk + 1])
+            worst[c] = max(worst[c], float(np.max(np.abs(X[: t.size][ok, j, i] - xi) / (1 + np.abs(xi)))))
+    calm = rho[idx] < 13.0
+    assert worst[calm].max() <= 1e-4, worst[calm].max()
+    assert np.median(worst) <= 1e-5, np.median(worst)
+
+
+@pytest.mark.parametrize("model", ["ornstein_uhlenbeck", "lactotroph_noise"])
+def test_seuler_fp32c_noise_words_equal_single_precision(gpu_available, model):
+    n = 300
+    if model == "ornstein_uhlenbeck":
+        x0, pars = np.zeros((1, n)), np.vstack([np.ones(n), np.zeros(n), 0.5 * np.ones(n)])
+    else:
+        x0 = np.tile([[-60.0], [0.0], [0.0], [0.1]], (1, n))
+        pars = np.vstack([1.5 * np.ones(n), 3.0 * np.ones(n), np.ones(n), 2.0 * np.ones(n)])
+    sp = (1.0 / 32, 1.0 / 32, 1e-6, 1e-3, 10 ** 8, 10, 1)  # float time sums of 1/32 are exact
+    p = Pair(model, "seuler", "thresh2", "fp32c", sp=sp, oracle=False)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(0.0, 200.0)
+    p.seed(77)
+    p.gpu.transient()
+    p.gpu.shift_x0()
+    p.gpu.features()
+    words = np.array(p.gpu.get_rng_state())
+    assert np.all(words != 77 + np.arange(2 * n, dtype=np.uint64))  # every member drew
+    for prec in ("single", "double"):
+        v, pp, a, nn = MODEL_INFO[model]
+        o = OracleEnsemble(model, len(v), len(pp), len(a), nn, stepper="seuler", observer="thresh2", precision=prec)
+        o.set_solver_params(*sp)
+        o.set_observer_params(x_up_threshold=0.3, x_down_threshold=0.2)
+        o.set_problem_data(x0, pars)
+        o.set_tspan((0.0, 200.0))
+        o.seed_rng(77)
+        o.transient()
+        o.shift_x0()
+        o.features()
+        same = words == o.get_rng()
+        if prec == "single":
+            assert same.all(), f"{int((~same).sum())} words differ from the single-precision oracle"
+        else:
+            # the double polar method can reject a pair the float one accepts
+            assert same.mean() >= 0.99
+            xf = np.array(p.gpu.get_xf())
+            assert np.max(np.abs(xf - o.get("xf")) / (1 + np.abs(o.get("xf")))) < 1e-4
+
+
+def test_van_der_pol_4096_full_size_bitexact(gpu_available):
+    """The reference's CPU-runnable configuration at full size: 4096 Van der
+    Pol members, mu linspace(0.1, 5), rk4 dt 0.01 FP64, transient over 1000,
+    x0 <- xf, thresh2 (warm-up + features) over 1000, every member against
+    the oracle bit for bit."""
+    n = 4096
+    mu = 0.1 + 4.9 * np.arange(n) / (n - 1)
+    x0 = np.vstack([2.0 * np.ones(n), np.zeros(n)])
+    p = Pair("van_der_pol", "rk4", "thresh2", "double", sp=(0.01, 0.01, 1e-6, 1e-3, 10 ** 8, 10, 1))
+    p.set_problem_data(x0, mu[None, :])
+    p.set_tspan(0.0, 1000.0)
+    p.transient()
+    p.assert_bitexact(("xf", "tf"))
+    p.shift_x0()
+    p.features(True)
+    p.assert_bitexact(("F", "xf", "tf"))
+    F = p.gpu_get("F").reshape(-1, n)
+    names = list(p.gpu.get_feature_names())
+    assert np.all(F[names.index("event count")] >= 2)  # every member oscillates
+
+
+def test_step_and_trial_counts_match_oracle(gpu_available):
+    """The device's per-member step and trial counters (the roofline's
+    denominators) against the oracle's, pass by pass: FP64 trial sequences
+    are identical, so the counts are equal."""
+    n = 150
+    g = np.linspace(0.0, 10.0, n)
+    x0 = np.tile(np.array([[-60.0], [0.0], [0.0], [0.1]]), (1, n))
+    pars = np.vstack([1.5 * np.ones(n), 3.0 * np.ones(n), np.linspace(2, 0, n), g])
+    p = Pair("lactotroph", "dopri5", "thresh2", "double", sp=(0.1, 1.0, 1e-8, 1e-6, 10 ** 8, 10, 1))
+    p.set_problem_data(x0, pars)
+    p.gpu.set_step_counter(True)
+    p.set_tspan(0.0, 1000.0)
+    for name, run in (("transient", p.transient), ("warm-up", p.initialize_observer), ("features", p.features)):
+        p.gpu.reset_step_counts()
+        run()
+        steps = int(np.sum(p.gpu.get_step_counts()))
+        trials = int(np.sum(p.gpu.get_trial_counts()))
+        assert (steps, trials) == (p.orc.last_steps, p.orc.last_trials), name
+        assert trials > steps  # rejected trials happen on this workload
+        if name == "transient":
+            p.shift_x0()

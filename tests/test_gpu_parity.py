@@ -331,3 +331,43 @@ def test_continuation_equals_one_long_run(observer):
     split.assert_bitexact(("F", "xf", "tf"))
     assert np.array_equal(split.gpu_get("F"), whole.gpu_get("F"))
     assert np.array_equal(split.gpu_get("tf") + 17.0, whole.gpu_get("tf"))
+
+
+@pytest.mark.parametrize("model,stepper,prec", [("lactotroph", "dopri5", "double"), ("lactotroph", "dopri5", "fp32c"),
+                                                ("lactotroph_noise", "seuler", "double"),
+                                                ("lactotroph_noise", "seuler", "fp32c")])
+def test_transient_features_one_launch(model, stepper, prec):
+    """clode_transient_features (transient, x0 <- xf, warm-up and features of
+    each member back to back in one launch) against the separate calls
+    transient(), shift_x0(), features(true) with the same spans: features,
+    states, step sizes, end times and noise words equal bit for bit; in FP64
+    both also equal the oracle."""
+    n = 130
+    sp = ADAPT if stepper == "dopri5" else (0.05, 1.0, 1e-6, 1e-3, 10 ** 7, 10, 1)
+    x0, pars = lac_data(n, noise=(model == "lactotroph_noise"))
+    out = []
+    for fused in (False, True):
+        print(f"transient_features {model} {prec} fused={fused}", flush=True)
+        p = Pair(model, stepper, "thresh2", prec, sp=sp, oracle=(prec == "double"))
+        p.set_problem_data(x0, pars)
+        p.seed(31)
+        if fused:
+            p.gpu.transient_features([0.0, 700.0], [0.0, 1500.0])
+            if p.orc:
+                p.orc.set_tspan((0.0, 700.0))
+                p.orc.transient()
+                p.orc.shift_x0()
+                p.orc.set_tspan((0.0, 1500.0))
+                p.orc.features(1)
+        else:
+            p.set_tspan(0.0, 700.0)
+            p.transient()
+            p.shift_x0()
+            p.set_tspan(0.0, 1500.0)
+            p.features(True)
+        if p.orc:
+            p.assert_bitexact(("F", "xf", "dt", "tf", "x0"))
+        assert tuple(p.gpu.get_tspan()) == (0.0, 1500.0)
+        out.append([p.gpu_get(k) for k in ("F", "xf", "x0", "dt", "tf")] + [np.array(p.gpu.get_rng_state())])
+    for a, b in zip(*out):
+        assert np.array_equal(a, b)

@@ -79,3 +79,40 @@ def test_pown_exact(n):
     got = cm_eval("pown", x, np.full_like(x, n))
     ref = np.array([float(np.float64(v) ** n) for v in x])
     assert np.allclose(got, ref, rtol=4e-16, atol=0)
+
+
+def root_reference(x, n):
+    """x^(1/n) in x87 extended precision (64-bit significand), rounded once to
+    double: within about 0.5 ulp of the exact root for every double x > 0."""
+    xl = x.astype(np.longdouble)
+    return np.exp(np.log(xl) / np.longdouble(n)).astype(np.float64)
+
+
+@pytest.mark.parametrize("n", [3, 5])
+def test_fixed_root_accuracy(n):
+    """cm::root<N> (the controller's (rtol/err)^(1/(p+1))) against an
+    extended-precision reference: within 1 ulp over the whole double range,
+    subnormals included, and against numpy's cube root for N = 3; the IEEE
+    special cases of pow(x, 1/N)."""
+    assert np.finfo(np.longdouble).nmant >= 63
+    rng = np.random.default_rng(40 + n)
+    x = np.concatenate([10 ** rng.uniform(-307, 308, 200000), rng.uniform(0.5, 2.0, 50000),
+                        2.0 ** rng.integers(-1074, 1023, 5000).astype(np.float64),
+                        rng.uniform(1, 1000, 2000) * 5e-324])
+    got = cm_eval(f"root{n}", x)
+    err = ulp_err64(got, root_reference(x, n))
+    assert err.max() <= 1.0, (n, float(err.max()), x[np.argmax(err)])
+    assert np.mean(err <= 0.5) > 0.85  # correctly rounded for most x
+    # pow(x, RN(1/N)) is further from the root: its exponent is rounded
+    assert ulp_err64(np.power(x, 1.0 / n), root_reference(x, n)).max() > 10
+    if n == 3:
+        assert ulp_err64(got, np.cbrt(x)).max() <= 2.0
+    # exact powers (k^N < 2^53): (k^N)^(1/N) = k
+    k = np.arange(1, 2000 if n == 3 else 1500, dtype=np.float64)
+    assert np.array_equal(cm_eval(f"root{n}", k ** n), k)
+    sp = cm_eval(f"root{n}", np.array([0.0, np.inf, np.nan, -1.0, -np.inf]))
+    assert sp[0] == 0.0 and sp[1] == np.inf and np.isnan(sp[2]) and np.isnan(sp[3]) and np.isnan(sp[4])
+    # float: the double root rounded once
+    xf = (10 ** rng.uniform(-37, 38, 20000)).astype(np.float32).astype(np.float64)
+    gotf = cm_eval(f"root{n}", xf, precision="single")
+    assert ulp_err32(gotf, root_reference(xf, n)).max() <= 1.0
