@@ -1112,6 +1112,21 @@ struct Observer {
     CLODE_DEV void finalizeObserverData(timetype ti, timetype t0)
     {
         timetype T = ti - t0;
+#if CLODE_PREC == 3
+        // The next update measures its step from the unshifted t_last = ti,
+        // so the running mean's recurrence m' E' = m (E' - dt') + x dt' weighs
+        // the mean so far by E' - dt' = E + T (E = ti - t_start, the time
+        // covered so far), not by E.  The integral (m E) takes the same
+        // factor, so that it keeps tracking m' E'.
+        const double E = ti - t_start;
+        if (E > 0.0) {
+            const double s = (E + T) / E;
+            for (int j = 0; j < CLODE_BASIC_NV; ++j)
+                xMean[j] *= s;
+            for (int j = 0; j < CLODE_BASIC_NA; ++j)
+                auxMean[j] *= s;
+        }
+#endif
         t_start -= T;
     }
 };

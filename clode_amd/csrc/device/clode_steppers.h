@@ -289,7 +289,7 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
 {
     realtype tNew, normErr, err[N_VAR], newxi[N_VAR], newk1[N_VAR];
     realtype threshold = sp.abstol / sp.reltol;
-    realtype hmin = RCONST(16.0) * fabs(fabs(nextafter(ti, RCONST(1.1) * t1)) - ti);
+    realtype hmin = cm::min_step(ti, RCONST(1.1) * t1);  // 16 |fabs(nextafter(ti, 1.1 t1)) - ti|
     if (!c.inStep) {
         c.inStep = true;
         c.noFailedSteps = true;
@@ -529,7 +529,7 @@ CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k
 {
     double tNew, normErr;
     realtype err[N_VAR], newxi[N_VAR], newk1[N_VAR], dx[N_VAR];
-    double hmin = 16.0 * fabs(fabs(nextafter(ti, 1.1 * t1)) - ti);
+    double hmin = cm::min_step(ti, 1.1 * t1);  // 16 |fabs(nextafter(ti, 1.1 t1)) - ti|
     if (!c.inStep) {
         c.inStep = true;
         c.noFailedSteps = true;

@@ -215,25 +215,30 @@ CM_FN double root(double x)
     const double inf = __builtin_huge_val();
     double xs = (x > 0.0 && x < inf) ? x : 1.0;
     int e;
-    double m = frexp(xs, &e);  // xs = m 2^e, m in [0.5, 1)
-    int q = e >= 0 ? e / N : -((N - 1 - e) / N);  // floor(e / N)
-    int r = e - N * q;                            // 0 .. N-1
+    double m = frexp(xs, &e);  // xs = m 2^e, m in [0.5, 1), e in [-1073, 1024]
+    // floor(e / N) and e mod N without a sign test: 1080 is a multiple of 3
+    // and of 5 and keeps e + 1080 positive
+    const unsigned eb = (unsigned)(e + 1080), qb = eb / N;
+    const int q = (int)qb - 1080 / N;
+    const int r = (int)(eb - N * qb);  // 0 .. N-1
     double M = ldexp(m, r);
     // w0 ~ m^(-1/N) on [1/2, 1) (least-squares cubic, within 3e-4 relative),
-    // times 2^(-r/N)
-    double w;
+    // times 2^(-r/N), picked by selects (no branches)
+    double w, sc = 1.0;
     if (N == 3) {
         w = fma(fma(fma(-0x1.f8703efde20eap-2, m, 0x1.93be6ca766c32p+0), m, -0x1.02f68e33ca1c6p+1), m,
                 0x1.f03c9f32afbc9p+0);
-        const double c1 = 0x1.965fea53d6e3dp-1, c2 = 0x1.428a2f98d728bp-1;  // 2^(-1/3), 2^(-2/3)
-        w *= (r == 0) ? 1.0 : ((r == 1) ? c1 : c2);
+        sc = (r == 1) ? 0x1.965fea53d6e3dp-1 : sc;  // 2^(-1/3)
+        sc = (r == 2) ? 0x1.428a2f98d728bp-1 : sc;  // 2^(-2/3)
     } else {
         w = fma(fma(fma(-0x1.ee2841a748840p-3, m, 0x1.91ce75060d2cfp-1), m, -0x1.0d5cc4c4abae0p+0), m,
                 0x1.82339d4c4e107p+0);
-        const double c1 = 0x1.bdb8cdadbe120p-1, c2 = 0x1.8406003b2ae5cp-1, c3 = 0x1.51cb453b9536cp-1,
-                     c4 = 0x1.2611186bae674p-1;  // 2^(-1/5) .. 2^(-4/5)
-        w *= (r == 0) ? 1.0 : ((r == 1) ? c1 : ((r == 2) ? c2 : ((r == 3) ? c3 : c4)));
+        sc = (r == 1) ? 0x1.bdb8cdadbe120p-1 : sc;  // 2^(-1/5)
+        sc = (r == 2) ? 0x1.8406003b2ae5cp-1 : sc;  // 2^(-2/5)
+        sc = (r == 3) ? 0x1.51cb453b9536cp-1 : sc;  // 2^(-3/5)
+        sc = (r == 4) ? 0x1.2611186bae674p-1 : sc;  // 2^(-4/5)
     }
+    w *= sc;
     // two Newton steps for w = M^(-1/N): relative error 3e-4 -> 5e-7 -> 2e-12
     const double rN = 1.0 / N;  // folded: RN(1/N)
     for (int it = 0; it < 2; ++it) {
@@ -259,6 +264,34 @@ template <int N>
 CM_FN float root(float x)
 {
     return (float)root<N>((double)x);
+}
+
+// The adaptive controller's minimum step, 16 |fabs(nextafter(t, 1.1 t1)) - t|
+// (1.1 t1 given as `toward`), exactly.  For a normal t > 0 below the largest
+// finite value, stepping toward a larger value, nextafter adds the spacing of
+// the binade, 2^(ilogb(t) - 52), so the result is 2^(ilogb(t) - 48): an
+// exponent read and a scale instead of nextafter's integer sequence.  Other
+// arguments (t <= 0, subnormal, the largest finite value, a step downward,
+// NaN) take the formula itself.
+CM_FN double min_step(double t, double toward)
+{
+    if (t >= 0x1p-1022 && t < 0x1.fffffffffffffp+1023 && t < toward) {
+        int e;
+        (void)frexp(t, &e);  // t = m 2^e, m in [1/2, 1): ilogb(t) = e - 1
+        return ldexp(1.0, e - 49);
+    }
+    return 16.0 * fabs(fabs(nextafter(t, toward)) - t);
+}
+
+// float: spacing 2^(ilogb(t) - 23), so 16 spacings are 2^(ilogb(t) - 19)
+CM_FN float min_step(float t, float toward)
+{
+    if (t >= 0x1p-126f && t < 0x1.fffffep+127f && t < toward) {
+        int e;
+        (void)frexpf(t, &e);
+        return ldexpf(1.0f, e - 20);
+    }
+    return 16.0f * fabsf(fabsf(nextafterf(t, toward)) - t);
 }
 
 // ----------------------------------------------------------------- float --

@@ -377,6 +377,12 @@ public:
         check(clode_last_kernel_ms(sim_, &ms));
         return ms;
     }
+    int last_launch_variant()
+    {
+        int w = 0;
+        check(clode_last_launch_variant(sim_, &w));
+        return w;
+    }
     int get_precision()
     {
         int p = 0;
@@ -834,6 +840,7 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("set_member_tspan", &SimulatorBasePy::set_member_tspan)
         .def("set_dt_refresh", &SimulatorBasePy::set_dt_refresh)
         .def("last_kernel_ms", &SimulatorBasePy::last_kernel_ms)
+        .def("last_launch_variant", &SimulatorBasePy::last_launch_variant)
         .def("set_step_counter", &SimulatorBasePy::set_step_counter)
         .def("reset_step_counts", &SimulatorBasePy::reset_step_counts)
         .def("get_step_counts", &SimulatorBasePy::get_step_counts)

@@ -722,6 +722,11 @@ int clode_last_kernel_ms(clode_sim *sim, double *ms)
     CHECK_SIM(sim) CLODE_TRY *need(ms, "ms") = sim->s->lastKernelMs;
     CLODE_CATCH
 }
+int clode_last_launch_variant(clode_sim *sim, int *waves_per_simd)
+{
+    CHECK_SIM(sim) CLODE_TRY *need(waves_per_simd, "waves_per_simd") = sim->s->lastVariantWaves;
+    CLODE_CATCH
+}
 int clode_set_step_counter(clode_sim *sim, int enable)
 {
     CHECK_SIM(sim) CLODE_TRY sim->s->setStepCounter(enable != 0);

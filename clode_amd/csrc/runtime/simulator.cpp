@@ -152,30 +152,35 @@ std::shared_ptr<Program> Simulator::currentProgram()
     return program;
 }
 
-std::shared_ptr<Program> Simulator::occupancyProgram()
+std::shared_ptr<Program> Simulator::occupancyProgram(int waves)
 {
     ProgramConfig c = programConfig();
-    c.wavesPerEU = 2;
+    c.wavesPerEU = waves;
     std::string key = compose_source(c);
-    if (!programW2 || key != programW2Key) {
-        programW2 = get_program(c, root.empty() ? std::string() : root);
-        programW2Key = key;
+    auto &slot = programsW[waves];
+    if (!slot.second || key != slot.first) {
+        slot.second = get_program(c, root.empty() ? std::string() : root);
+        slot.first = key;
     }
-    return programW2;
+    return slot.second;
 }
 
 // Kernel for a grid launch of nThreads members.  A kernel whose register
-// footprint allows one wave per SIMD leaves each SIMD without a second wave
-// to switch to while its one wave waits; when the ensemble has at least two
-// waves of members per SIMD, the variant compiled for two waves per SIMD
-// (registers capped at 256, the rest spilled) is used instead.  At exactly
-// one wave per SIMD it gains nothing (lactotroph dopri5 features: 65,536
-// members equal, 131,072 6 % faster in FP64 and 11 % in fp32c).  Both
-// variants perform the same arithmetic in the same order.
+// footprint allows w waves per SIMD leaves each SIMD only w waves to switch
+// between while one waits on a dependency.  When the ensemble has at least
+// w + 1 waves of members per SIMD and w < 4, the variant compiled for w + 1
+// waves per SIMD (registers capped at 512 / (w + 1), the rest spilled) is
+// used instead.  Measured gains: lactotroph dopri5 features 1 -> 2 waves,
+// 6 % (FP64) and 11 % (fp32c) at 131,072 members, nothing at 65,536; Lorenz
+// localmax features on 2^20 members, 2 -> 3 waves in FP64, 6 % (dopri5) and
+// 7 % (bs23); 3 -> 4 waves in fp32c, 5 % and 7 %.  Forcing one more wave
+// than that (FP64 Lorenz dopri5 at 4 waves) spills enough to lose 26 %.
+// Every variant performs the same arithmetic in the same order.
 // CLODE_OCCUPANCY_VARIANT=0 in the environment keeps the default kernel.
 hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads, unsigned int block)
 {
     hipFunction_t fn = currentProgram()->function(device, kernel);
+    lastVariantWaves = 0;
     static const bool enabled = [] {
         const char *e = std::getenv("CLODE_OCCUPANCY_VARIANT");
         return !(e && std::strcmp(e, "0") == 0);
@@ -185,13 +190,13 @@ hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads,
     int cus = 0, perCU = 0;
     hip_check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
     const size_t simds = (size_t)std::max(cus, 1) * 4;
-    if ((size_t)nThreads < simds * 64 * 2)
-        return fn;
     hip_check(hipModuleOccupancyMaxActiveBlocksPerMultiprocessor(&perCU, fn, (int)block, 0),
               "hipModuleOccupancyMaxActiveBlocksPerMultiprocessor");
-    if ((size_t)perCU * block / 64 > 4)  // more than one wave per SIMD already
+    const int waves = (int)((size_t)perCU * block / 64 / 4);  // resident waves per SIMD
+    if (waves < 1 || waves >= 4 || (size_t)nThreads < simds * 64 * (size_t)(waves + 1))
         return fn;
-    return occupancyProgram()->function(device, kernel);
+    lastVariantWaves = waves + 1;
+    return occupancyProgram(waves + 1)->function(device, kernel);
 }
 
 // CLODE::buildCL (CLODE.cpp:155-171) + the subclasses' kernel creation
@@ -518,6 +523,7 @@ void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThr
             d_work.allocate(device, 256);
         args.work_counter = d_work.as<unsigned int>();
         hipFunction_t reset = p->function(device, "clode_queue_reset");
+        lastVariantWaves = 0;
         unsigned int one = 1;
         void *rparams[] = {&args.work_counter, &one};
         hip_check(hipModuleLaunchKernel(reset, 1, 1, 1, 64, 1, 1, 0, stream, rparams, nullptr),

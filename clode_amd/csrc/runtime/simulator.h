@@ -11,6 +11,8 @@
 // re-initialisation is requested, and every compute call blocks.
 #pragma once
 
+#include <map>
+
 #include "clode_runtime.h"
 
 namespace clode {
@@ -96,6 +98,9 @@ public:
     unsigned int queueSlice = 8;  // work queue: loop iterations between queue checks
     bool dtRefresh = false;
     double lastKernelMs = 0.0;
+    // waves per SIMD the last launch's kernel was compiled for (0: the
+    // default compilation)
+    int lastVariantWaves = 0;
     bool stepCounter = false;
 
     void setStepCounter(bool on);
@@ -122,11 +127,10 @@ private:
     int observerId() const;
     ProgramConfig programConfig() const;
     std::shared_ptr<Program> currentProgram();
-    // the same program compiled for two waves per SIMD, used by grid launches
-    // of large ensembles whose default kernel fits only one wave per SIMD
-    std::shared_ptr<Program> occupancyProgram();
-    std::shared_ptr<Program> programW2;
-    std::string programW2Key;
+    // the same program compiled for at least `waves` waves per SIMD, used by
+    // grid launches of large ensembles (see gridFunction)
+    std::shared_ptr<Program> occupancyProgram(int waves);
+    std::map<int, std::pair<std::string, std::shared_ptr<Program>>> programsW;
     hipFunction_t gridFunction(const char *kernel, unsigned int nThreads, unsigned int block);
     void ensureStream();
     void setNpts(int n);

@@ -421,6 +421,13 @@ class Simulator:
         """Duration of the last kernel launch (HIP events), milliseconds."""
         return self._integrator.last_kernel_ms()
 
+    def last_launch_variant(self) -> int:
+        """0 when the last launch ran the default compilation of the program;
+        w > 0 when it ran the variant compiled for at least w waves per SIMD
+        (grid launches of ensembles larger than the default kernel keeps
+        resident)."""
+        return self._integrator.last_launch_variant()
+
     def enable_step_counter(self, on: bool = True) -> None:
         """Count accepted steps per member over all launches (member-steps)."""
         self._integrator.set_step_counter(on)

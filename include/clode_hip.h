@@ -212,6 +212,11 @@ int clode_set_dt_refresh(clode_sim *sim, int enable);
 /* Time of the last kernel launch on the simulator's stream, milliseconds
  * (HIP events around the launch). */
 int clode_last_kernel_ms(clode_sim *sim, double *ms);
+/* Which compilation of the program the last launch used: 0 for the default,
+ * w > 0 for the variant compiled for at least w waves per SIMD, which grid
+ * launches of ensembles with more waves of members than the default kernel
+ * keeps resident per SIMD use (no reference counterpart: a diagnostic). */
+int clode_last_launch_variant(clode_sim *sim, int *waves_per_simd);
 /* Compile-only check of the current configuration (no device needed). */
 int clode_compile(clode_sim *sim);
 /* Per-member count of accepted steps (member-steps), accumulated over every

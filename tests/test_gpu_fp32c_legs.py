@@ -17,11 +17,17 @@ is FP64 at rtol against FP64 at rtol / 100 on the same members.  Stated bounds:
   rtol / 100 when that is larger: Lorenz's local maxima on a damped spiral
   are ill-conditioned (FP64 at rtol and at rtol / 100 disagree on about a
   third of those members, at rtol / 100 and rtol / 10^4 still on a sixth).
+  Statistics over events (intervals, amplitudes, durations, periods, event
+  times) are compared on the members whose event counts agree in all three
+  runs, at least 40 % of them: over different sets of events a maximum
+  interval is a different quantity.
   (Member by member the comparison is noisier: extrema over step-point
   event times, such as the longest period, move with the step grid.)  Ornstein-Uhlenbeck
   (Euler-Maruyama, no tolerance): the noise words equal the FP64 run's for
-  >= 99 % of the members and, on those, max / min / slopes agree to 1e-5
-  relative, xf to 1e-5 (1 + |x|) and the time mean to 1e-5 of the range.
+  >= 99 % of the members and, on those, max / min agree to 1e-5 relative,
+  the extreme slopes to 5e-5 (the float normals of the largest draws carry
+  ~1e-5 relative error from the polar method's 2 U - 1), xf to
+  1e-5 (1 + |x|) and the time mean to 1e-5 of the range.
   Lorenz uses rho in [0, 13) (no chaos) and short spans (median
   5), before the damped oscillation sinks below float resolution, where
   fp32c's localmax counts rounding noise as maxima (documented limitation).
@@ -122,6 +128,13 @@ def test_mixed_batch_fp32c_against_fp64(gpu_available):
                 if nm == "mean x":
                     span = B[names.index("max x")][same] - B[names.index("min x")][same]
                     assert np.all(np.abs(a - r) <= 1e-5 * span), nm
+                elif "/dt" in nm:
+                    # the slope holds the noise term, and the float polar draw
+                    # forms u = 2 U - 1 from a 24-bit uniform: for the largest
+                    # normals (|u|, |v| ~ 5e-3) that cancellation leaves
+                    # ~1e-5 relative error (2e-5 seen), which the extreme
+                    # slopes inherit
+                    assert np.all(np.abs(a - r) <= 5e-5 * np.abs(r) + 1e-12), nm
                 else:
                     assert np.all(np.abs(a - r) <= 1e-5 * np.abs(r) + 1e-12), nm
             continue
@@ -130,7 +143,13 @@ def test_mixed_batch_fp32c_against_fp64(gpu_available):
         miss, miss_ref = float(np.mean(A[ec] != B[ec])), float(np.mean(B[ec] != Bf[ec]))
         assert miss <= max(0.02, 1.5 * miss_ref), (model, miss, miss_ref)
         excess = {}
-        rows = [(nm, A[j], B[j], Bf[j]) for j, nm in enumerate(names)]
+        # statistics over events (intervals, amplitudes, durations, event
+        # times) compare only where all three runs saw the same events
+        same_events = (A[ec] == B[ec]) & (B[ec] == Bf[ec])
+        assert same_events.mean() >= 0.4, (model, float(same_events.mean()))
+        per_event = ("IMI", "amplitude", "period", "peaks", "Duration", "duty", "event time", "event evar")
+        rows = [(nm, A[j], B[j], Bf[j]) if not any(w in nm for w in per_event)
+                else (nm, A[j][same_events], B[j][same_events], Bf[j][same_events]) for j, nm in enumerate(names)]
         rows += [(f"xf[{j}]", X[j], XB[j], XBf[j]) for j in range(X.shape[0])]
         for nm, a, r, rf in rows:
             d, t = np.abs(a - r), np.abs(r - rf)

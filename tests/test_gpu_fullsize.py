@@ -101,3 +101,60 @@ def test_lorenz_2p20_fp32c(gpu_available):
             assert np.mean(within) >= 0.95, (key, float(np.mean(within)))
             assert np.all(err <= 0.05), (key, float(err.max()))
         assert float(np.median(err / np.abs(b))) < 1e-4, key
+
+
+@pytest.mark.parametrize("stepper", ["dopri5", "bs23"])
+def test_lorenz_2p20_fp64_occupancy_variant_sample_bitexact(gpu_available, stepper):
+    """2^20 members are 16 waves per SIMD, more than the FP64 localmax
+    features kernel keeps resident, so the grid launch runs the variant
+    compiled for one more wave per SIMD; it must give the oracle's results
+    bit for bit on a sample of members, like the default kernel."""
+    n = 1 << 20
+    rng = np.random.default_rng(8)
+    x0 = rng.normal(1.0, 0.1, size=(n, 3))
+    rho = np.linspace(0.0, 250.0, n)
+    st = clode.Stepper.dormand_prince if stepper == "dopri5" else clode.Stepper.bs23
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 1.0, "z": 1.0},
+                                 parameters={"sigma": 10.0, "rho": 28.0, "beta": 8 / 3},
+                                 observer=clode.Observer.local_max, stepper=st,
+                                 single_precision=False, dt=0.01, dtmax=1.0, abstol=1e-8, reltol=1e-6,
+                                 max_steps=10 ** 8, t_span=(0.0, 20.0))
+    sim.set_ensemble(variables=x0, parameters={"rho": rho})
+    out = sim.features()
+    assert sim.last_launch_variant() >= 2, "the occupancy variant did not run"
+    F = out.to_ndarray().reshape(n, -1)
+    idx = np.arange(3, n, 2039)
+    o = OracleEnsemble("lorenz", 3, 3, stepper=stepper, observer="localmax", precision="double")
+    o.set_solver_params(0.01, 1.0, 1e-8, 1e-6, 10 ** 8, 1000000, 1)
+    o.set_observer_params()
+    o.set_problem_data(x0[idx].T, np.vstack([10.0 * np.ones(idx.size), rho[idx], np.full(idx.size, 8 / 3)]))
+    o.set_tspan((0.0, 20.0))
+    o.features()
+    assert np.array_equal(F[idx], o.get("F").reshape(-1, idx.size).T)
+
+
+def test_lactotroph_2p17_occupancy_variant_sample_bitexact(gpu_available):
+    """131,072 lactotroph members are two waves per SIMD; the FP64 thresh2
+    features kernel keeps one resident, so the two-wave variant runs, bit for
+    bit with the oracle on a sample."""
+    sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lactotroph.cl"),
+                                 variables={"v": -60.0, "n": 0.0, "f": 0.0, "c": 0.1},
+                                 parameters={"gcal": 1.5, "gsk": 3.0, "gbk": 1.0, "gk": 2.0}, aux=["ica"],
+                                 observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                 single_precision=False, dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6,
+                                 max_steps=10 ** 8, t_span=(0.0, 300.0), observer_x_up_thresh=0.3,
+                                 observer_x_down_thresh=0.2)
+    GK, GBK = np.meshgrid(np.linspace(0, 10, 512), np.linspace(0, 2, 256), indexing="ij")
+    sim.set_ensemble(parameters={"gk": GK, "gbk": GBK})
+    out = sim.features()
+    assert sim.last_launch_variant() >= 2, "the occupancy variant did not run"
+    F = out.to_ndarray().reshape(GK.size, -1)
+    idx = np.arange(11, GK.size, 509)
+    o = OracleEnsemble("lactotroph", 4, 4, 1, stepper="dopri5", observer="thresh2", precision="double")
+    o.set_solver_params(0.1, 1.0, 1e-8, 1e-6, 10 ** 8, 1000000, 1)
+    o.set_observer_params(x_up_threshold=0.3, x_down_threshold=0.2)
+    pars = np.vstack([1.5 * np.ones(idx.size), 3.0 * np.ones(idx.size), GBK.ravel()[idx], GK.ravel()[idx]])
+    o.set_problem_data(np.tile([[-60.0], [0.0], [0.0], [0.1]], (1, idx.size)), pars)
+    o.set_tspan((0.0, 300.0))
+    o.features()
+    assert np.array_equal(F[idx], o.get("F").reshape(-1, idx.size).T)

@@ -116,3 +116,31 @@ def test_fixed_root_accuracy(n):
     xf = (10 ** rng.uniform(-37, 38, 20000)).astype(np.float32).astype(np.float64)
     gotf = cm_eval(f"root{n}", xf, precision="single")
     assert ulp_err32(gotf, root_reference(xf, n)).max() <= 1.0
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_min_step_is_sixteen_spacings(precision):
+    """cm::min_step(t, 1.1 t1), the controller's minimum step, equals the
+    formula 16 |fabs(nextafter(t, 1.1 t1)) - t| it replaces, bit for bit, over
+    normal, subnormal, zero, negative, extreme and non-finite times and both
+    directions (numpy's nextafter is the independent side)."""
+    rng = np.random.default_rng(77)
+    ft = np.float64 if precision == "double" else np.float32
+    fi = np.finfo(ft)
+    mag = 10 ** rng.uniform(-300 if ft is np.float64 else -37, 300 if ft is np.float64 else 37, 50000)
+    t = np.concatenate([mag, -mag[:5000], rng.uniform(0, 1000, 20000),
+                        2.0 ** rng.integers(-200, 100, 5000).astype(np.float64),
+                        [0.0, -0.0, float(fi.tiny), float(fi.tiny) / 4, float(fi.max), -float(fi.max),
+                         float(fi.smallest_subnormal), np.inf, -np.inf, np.nan]]).astype(ft)
+    t1 = np.concatenate([t[:-10] * ft(1) + np.abs(rng.normal(0, 1, t.size - 10)).astype(ft) * np.abs(t[:-10]),
+                         rng.uniform(-10, 10, 10).astype(ft)]).astype(ft)
+    t1[::7] = t[::7] * ft(0.5)  # toward a smaller value on a seventh of the cases
+    toward = (ft(1.1) * t1).astype(ft)
+    with np.errstate(all="ignore"):
+        want = (ft(16) * np.abs(np.abs(np.nextafter(t, toward)) - t)).astype(ft)
+        got = cm_eval("min_step", t.astype(np.float64), toward.astype(np.float64), precision=precision)
+    same = (got.astype(ft) == want) | (np.isnan(got) & np.isnan(want))
+    assert same.all(), (t[~same][:5], toward[~same][:5], got[~same][:5], want[~same][:5])
+    # the fast branch covers the usual case: a positive time short of t1
+    usual = (t > fi.tiny) & (t < toward) & np.isfinite(t)
+    assert usual.mean() > 0.5
