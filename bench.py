@@ -39,7 +39,7 @@ REPO = os.path.dirname(os.path.abspath(__file__))
 MODELS = os.path.join(REPO, "clode_amd", "models")
 
 
-def lactotroph_sim(precision, n_side=256):
+def lactotroph_sim(precision, n_side=256, rtol=1e-6, atol=1e-8):
     import clode_amd as clode
 
     sim = clode.FeatureSimulator(
@@ -50,7 +50,7 @@ def lactotroph_sim(precision, n_side=256):
         observer=clode.Observer.threshold_2,
         stepper=clode.Stepper.dormand_prince,
         single_precision=precision,
-        dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6, max_steps=100000000,
+        dt=0.1, dtmax=1.0, abstol=atol, reltol=rtol, max_steps=100000000,
         t_span=(0.0, 5000.0),
         observer_x_up_thresh=0.3, observer_x_down_thresh=0.2,
     )
@@ -98,6 +98,7 @@ def run_lactotroph(precision, steps, warmup):
         trials.append(t - sum(trials))
 
     sim.reset_step_counts()
+    x0_step = sim.get_initial_state().copy()
     sim.transient(t_span=(0.0, 5000.0))
     take()
     k_tr = sim.last_kernel_ms()
@@ -119,7 +120,21 @@ def run_lactotroph(precision, steps, warmup):
         "kernel_trials": {"transient": trials[0], "warmup": trials[1], "features": trials[2]},
         "features": out.to_ndarray(),
         "feature_names": out.get_feature_names(),
+        "x0_step": x0_step,
     }
+
+
+def lactotroph_step_features(precision, x0, rtol=1e-6, atol=1e-8):
+    """Features of one bench step (transient, warm-up, features) of a fresh
+    simulator from the initial states x0: the fp32c error study runs FP64,
+    fp32c and FP64 at rtol / 100 (the yardstick of the method's own
+    truncation error) from the same states and the same first step size."""
+    sim = lactotroph_sim(precision, rtol=rtol, atol=atol)
+    gk, gbk = np.linspace(0.0, 10.0, 256), np.linspace(0.0, 2.0, 256)
+    GK, GBK = np.meshgrid(gk, gbk, indexing="ij")
+    sim.set_ensemble(variables=x0, parameters={"gk": GK, "gbk": GBK})
+    one_step(sim)
+    return sim.get_observer_results().to_ndarray()
 
 
 def one_launch_sequence(precision, steps, r_three=None):
@@ -160,9 +175,12 @@ def kernel_tflops(r, phase, workload="lactotroph_256x256_dopri5_thresh2"):
     return fl / (r["kernel_ms"][phase] / 1e3) / 1e12, fl
 
 
-def feature_error(f32c, f64, names):
-    """Relative error of every fp32c feature against FP64 over the map's
-    members: median, p99, the fraction within 1e-4 and the fraction beyond 1 %."""
+def feature_error(f32c, f64, names, fine=None):
+    """Relative error of every fp32c feature against FP64 over the members:
+    median, p99, the fraction within 1e-4 and the fraction beyond 1 %.  With
+    `fine` (FP64 at rtol / 100 on the same members), also the same figures
+    for FP64's own truncation error |fp64 - fine| and the fraction of members
+    on which fp32c is within 3x of it (plus 1e-5 (1 + |fp64|))."""
     res = {}
     for key in names:
         j = names.index(key)
@@ -176,6 +194,13 @@ def feature_error(f32c, f64, names):
             "frac_le_1e-4": float(np.mean(rel[valid] <= 1e-4)) if valid.any() else 1.0,
             "frac_gt_1pct": float(np.mean(rel[valid] > 1e-2)) if valid.any() else 0.0,
         }
+        if fine is not None:
+            t = np.abs(b - fine[:, j])
+            trel = t / scale
+            res[key]["fp64_truncation_median_rel"] = float(np.median(trel[valid])) if valid.any() else 0.0
+            res[key]["fp64_truncation_p99_rel"] = float(np.percentile(trel[valid], 99)) if valid.any() else 0.0
+            res[key]["frac_within_3x_fp64_truncation"] = float(
+                np.mean(np.abs(a - b) <= 3 * t + 1e-5 * (1 + np.abs(b))))
     return res
 
 
@@ -336,7 +361,11 @@ def noise_c2(n=65536, seed=2024):
 def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
     """Lorenz, 2^20 members (sigma 10, beta 8/3, rho in [0, 250], x0 ~ N(1, 0.1)),
     localmax on x over (0, 500), dopri5 and bs23 at rtol 1e-6 atol 1e-8, FP64
-    and fp32c; then the trajectory of the first 65,536 members, 1024 samples
+    and fp32c: kernel time and algorithmic FLOP rate of the features launch,
+    wall time of a repeated features() call with the feature readback (the
+    first call, which loads the program and allocates, is reported apart),
+    and every fp32c feature's error against FP64 on the rho < 13 members;
+    then the trajectory of the first 65,536 members, 1024 samples
     each (nout 1), float state, with its HBM write rate, and the same number of
     bytes from all 2^20 members at 64 samples each."""
     import clode_amd as clode
@@ -348,20 +377,34 @@ def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
     common = dict(src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 1.0, "z": 1.0},
                   parameters={"sigma": 10.0, "rho": 28.0, "beta": 8.0 / 3.0}, dt=0.01, dtmax=1.0, abstol=1e-8,
                   reltol=1e-6, max_steps=100000000, t_span=(0.0, t_end))
+    from clode_amd.perfmodel import PEAK_TFLOPS, launch_flops
+
     for stepper in (clode.Stepper.dormand_prince, clode.Stepper.bs23):
         r = {}
+        workload = f"lorenz_2p20_{stepper.value}_localmax"
         for prec in ("double", "fp32c"):
             sim = clode.FeatureSimulator(observer=clode.Observer.local_max, stepper=stepper, single_precision=prec,
                                          **common)
             sim.set_ensemble(variables=x0, parameters={"rho": rho})
+            # first call: device buffers, program and variant loads; untimed
+            # for the rate, its features are the ones compared below
+            t0 = time.perf_counter()
+            first = sim.features(update_x0=False, fetch_results=True).to_ndarray()
+            first_wall = time.perf_counter() - t0
             sim.enable_step_counter(True)
             t0 = time.perf_counter()
-            out = sim.features(fetch_results=True)
+            out = sim.features(initialize_observer=True, update_x0=False, fetch_results=True)
             wall = time.perf_counter() - t0
-            steps = int(sim.get_step_counts().sum())
-            r[prec] = {"kernel_ms": sim.last_kernel_ms(), "wall_s": wall, "member_steps": steps,
-                       "member_steps_per_s": steps / wall}
-            r[prec + "_F"] = out.to_ndarray()
+            steps, trials = int(sim.get_step_counts().sum()), int(sim.get_trial_counts().sum())
+            ms = sim.last_kernel_ms()
+            fl = launch_flops(workload, "features", steps, trials)
+            tf = fl / (ms / 1e3) / 1e12
+            peak = PEAK_TFLOPS["f64"] if prec == "double" else PEAK_TFLOPS["f32"]
+            r[prec] = {"kernel_ms": ms, "wall_s": wall, "first_call_wall_s": first_wall, "member_steps": steps,
+                       "trials": trials, "member_steps_per_s": steps / wall, "algorithmic_tflops": tf,
+                       "frac_of_peak": tf / peak, "peak_tflops": peak,
+                       "occupancy_variant_waves_per_simd": sim.last_launch_variant()}
+            r[prec + "_F"] = first
             r["names"] = out.get_feature_names()
         # fp32c feature error on the members without chaotic transients
         # (rho < 13), against FP64 at the same tolerance; the yardstick is the
@@ -371,20 +414,12 @@ def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
                                           **dict(common, abstol=1e-10, reltol=1e-8))
         fine_sim.set_ensemble(variables=x0[calm], parameters={"rho": rho[calm]})
         fine = fine_sim.features(fetch_results=True).to_ndarray()
-        err = {}
-        for key in ("mean x", "max z", "min y"):
-            j = r["names"].index(key)
-            a, b, f = r["fp32c_F"][calm, j], r["double_F"][calm, j], fine[:, j]
-            d = np.abs(a - b)
-            rel = d / np.maximum(np.abs(b), 1e-30)
-            err[key] = {"median_rel": float(np.median(rel)), "p99_rel": float(np.percentile(rel, 99)),
-                        "frac_le_1e-4": float(np.mean(rel <= 1e-4)),
-                        "fp64_truncation_median_rel": float(np.median(np.abs(b - f) / np.maximum(np.abs(b), 1e-30))),
-                        "frac_within_3x_fp64_truncation": float(np.mean(d <= 3 * np.abs(b - f) + 1e-5 * (1 + np.abs(b))))}
-        r["fp32c_feature_error_rho_lt_13"] = err
+        r["fp32c_feature_error_rho_lt_13"] = feature_error(r["fp32c_F"][calm], r["double_F"][calm], r["names"], fine)
+        r["feature_error_members"] = int(calm.sum())
         for k in ("fp32c_F", "double_F", "names"):
             r.pop(k)
         r["fp32c_speedup_kernel"] = r["double"]["kernel_ms"] / r["fp32c"]["kernel_ms"]
+        r["fp32c_speedup_wall"] = r["double"]["wall_s"] / r["fp32c"]["wall_s"]
         res[stepper.value] = r
     sim = clode.TrajectorySimulator(stepper=clode.Stepper.dormand_prince, single_precision="fp32c",
                                     max_store=store, nout=1, **common)
@@ -672,7 +707,11 @@ def main():
             "member_steps_per_s": r32["member_steps_per_s"], "ms_per_step": r32["ms_per_step"],
             "kernel_ms": r32["kernel_ms"], "speedup_vs_fp64_wall": r64["ms_per_step"] / r32["ms_per_step"],
             "features_kernel_tflops": a32, "features_kernel_frac_of_fp32_peak": a32 / PEAK_TFLOPS["f32"],
-            "feature_error_vs_fp64": feature_error(r32["features"], r64["features"], r64["feature_names"]),
+            "feature_error_vs_fp64": feature_error(
+                lactotroph_step_features("fp32c", r64["x0_step"]),
+                lactotroph_step_features("double", r64["x0_step"]), r64["feature_names"],
+                lactotroph_step_features("double", r64["x0_step"], 1e-8, 1e-10)),
+            "feature_error_members": "all 65,536, one step from the same initial states (fresh simulators)",
         }
     if "seq" in wanted:
         run_extra("one_launch_sequence_fp64", lambda: one_launch_sequence("double", args.steps, r64))

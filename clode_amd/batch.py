@@ -59,11 +59,9 @@ class BatchSimulator:
         rows, cached = self._batch.autotune_table()
         return np.asarray(rows), bool(cached)
 
-    def _invalidate(self, features: bool = False, x0: bool = False) -> None:
+    def _invalidate(self, x0: bool = False) -> None:
         for s in self._sims:
             s._device_final_state = s._device_dt = s._device_tf = None
-            if features:
-                s._device_features = None
             if x0:
                 s._device_initial_state = None
 
@@ -83,7 +81,7 @@ class BatchSimulator:
         if not self._features:
             raise ValueError("features needs feature simulators")
         self._batch.features(-1 if initialize_observer is None else int(bool(initialize_observer)))
-        self._invalidate(features=True)
+        self._invalidate()
         if update_x0:
             self.shift_x0()
 
@@ -95,7 +93,7 @@ class BatchSimulator:
         """rounds x (transient, x0 <- xf, features, x0 <- xf); with graph=True
         the whole sequence is one captured graph launched once."""
         self._batch.sweep(rounds, graph)
-        self._invalidate(features=True, x0=True)
+        self._invalidate(x0=True)
 
     def last_kernel_ms(self) -> float:
         """Device time of the last batch operation, milliseconds."""

@@ -530,6 +530,27 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS
     run_member<InitObsLane>(s.fe, sp, i);
     run_member<FeaturesLane>(s.fe, sp, i);
 }
+
+// The feature matrix F ([nF][nPts], member index fastest, as the observers
+// write it) as double, member-major ([nPts][nF]): the layout the host API
+// hands out, transposed here instead of on the host.  A block moves a tile
+// of 64 members x 32 features through LDS, so that both the reads (along
+// members) and the writes (along features) are contiguous runs.
+extern "C" __global__ void __launch_bounds__(64) clode_f_members(const ostore *F, double *out, unsigned int nPts,
+                                                                unsigned int nF)
+{
+    __shared__ double tile[32][65];
+    const unsigned int m0 = blockIdx.x * 64u, f0 = blockIdx.y * 32u, t = threadIdx.x;
+    for (unsigned int k = 0; k < 32u; ++k)
+        if (f0 + k < nF && m0 + t < nPts)
+            tile[k][t] = (double)F[(size_t)(f0 + k) * nPts + m0 + t];
+    __syncthreads();
+    for (unsigned int L = t; L < 64u * 32u; L += 64u) {
+        const unsigned int r = L / 32u, c = L % 32u;
+        if (m0 + r < nPts && f0 + c < nF)
+            out[(size_t)(m0 + r) * nF + f0 + c] = tile[c][r];
+    }
+}
 #endif  // CLODE_OBSERVER
 #endif  // CLODE_BATCH
 

@@ -474,6 +474,17 @@ public:
         check(clode_get_f(sim_, v.mutable_data(), (size_t)v.size()));
         return v;
     }
+    // F as a C-ordered (members, features) array, transposed on the device
+    DArray get_f_members()
+    {
+        const py::ssize_t nf = get_n_features(), n = (py::ssize_t)n_pts();
+        DArray v({n, nf});
+        {
+            py::gil_scoped_release nogil;
+            check(clode_get_f_members(sim_, v.mutable_data(), (size_t)(n * nf)));
+        }
+        return v;
+    }
     std::vector<std::string> get_feature_names()
     {
         return split(get_string([&](char *b, size_t c, size_t *n) { return clode_get_feature_names(sim_, b, c, n); }));
@@ -862,6 +873,7 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("get_observer_params", &FeatureSimulatorBasePy::get_observer_params)
         .def("get_observer_name", &FeatureSimulatorBasePy::get_observer_name)
         .def("get_f", &FeatureSimulatorBasePy::get_f)
+        .def("get_f_members", &FeatureSimulatorBasePy::get_f_members)
         .def("get_n_features", &FeatureSimulatorBasePy::get_n_features)
         .def("get_feature_names", &FeatureSimulatorBasePy::get_feature_names)
         .def("get_available_observers", &FeatureSimulatorBasePy::get_available_observers)

@@ -485,7 +485,14 @@ int clode_get_n_features(clode_sim *sim, int *n)
 }
 int clode_get_f(clode_sim *sim, double *out, size_t n)
 {
-    CHECK_SIM(sim) CLODE_TRY return copy_vec(sim->s->getF(), out, n);
+    CHECK_SIM(sim) CLODE_TRY if (!out && n > 0) throw clode::InvalidArgument("null output buffer");
+    sim->s->getFInto(out, n);
+    CLODE_CATCH
+}
+int clode_get_f_members(clode_sim *sim, double *out, size_t n)
+{
+    CHECK_SIM(sim) CLODE_TRY if (!out && n > 0) throw clode::InvalidArgument("null output buffer");
+    sim->s->getFMembers(out, n);
     CLODE_CATCH
 }
 int clode_get_feature_names(clode_sim *sim, char *buf, size_t cap, size_t *needed)

@@ -268,6 +268,26 @@ std::vector<double> Simulator::download(const DeviceBuffer &b, size_t n, size_t 
     return out;
 }
 
+void Simulator::downloadInto(const DeviceBuffer &b, size_t n, size_t elem, double *out)
+{
+    ensureStream();
+    if (n == 0)
+        return;
+    if (!b.ptr) {
+        std::fill(out, out + n, 0.0);
+        return;
+    }
+    if (elem == 8) {
+        hip_check(hipMemcpyAsync(out, b.ptr, n * 8, hipMemcpyDeviceToHost, stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+    } else {
+        std::vector<float> f(n);
+        hip_check(hipMemcpyAsync(f.data(), b.ptr, n * 4, hipMemcpyDeviceToHost, stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+        std::copy(f.begin(), f.end(), out);
+    }
+}
+
 // CLODE::setNpts (CLODE.cpp:174-242)
 void Simulator::setNpts(int newNpts)
 {
@@ -758,6 +778,40 @@ std::vector<double> Simulator::getF()
     if (nPts <= 0 || !d_F.ptr)
         return std::vector<double>(Felements, 0.0);
     return download(d_F, Felements, wideSize());
+}
+
+void Simulator::getFInto(double *out, size_t cap)
+{
+    if (cap < Felements)
+        throw InvalidArgument("output buffer too small: need " + std::to_string(Felements));
+    if (nPts <= 0 || !d_F.ptr) {
+        std::fill(out, out + Felements, 0.0);
+        return;
+    }
+    downloadInto(d_F, Felements, wideSize(), out);
+}
+
+// The Python API's layout, F[member][feature] (features.py:467-507 reshapes
+// the reference's feature-major vector to it on the host).
+void Simulator::getFMembers(double *out, size_t cap)
+{
+    if (cap < Felements)
+        throw InvalidArgument("output buffer too small: need " + std::to_string(Felements));
+    if (nPts <= 0 || !d_F.ptr || Felements == 0) {
+        std::fill(out, out + Felements, 0.0);
+        return;
+    }
+    ensureStream();
+    if (!d_Fm.ptr || d_Fm.bytes != Felements * 8)
+        d_Fm.allocate(device, Felements * 8);
+    hipFunction_t fn = currentProgram()->function(device, "clode_f_members");
+    const void *src = d_F.ptr;
+    void *dst = d_Fm.ptr;
+    unsigned int n = (unsigned int)nPts, nF = (unsigned int)(Felements / (size_t)nPts);
+    void *params[] = {&src, &dst, &n, &nF};
+    hip_check(hipModuleLaunchKernel(fn, (n + 63u) / 64u, (nF + 31u) / 32u, 1, 64, 1, 1, 0, stream, params, nullptr),
+              "launch clode_f_members");
+    downloadInto(d_Fm, Felements, 8, out);
 }
 
 // ---------------------------------------------------------------- trajectory

@@ -61,6 +61,10 @@ public:
     void features(bool reinit);
     void transientFeatures(double tr0, double tr1, double fe0, double fe1);
     std::vector<double> getF();
+    // F into out (capacity cap): feature-major as getF, or member-major
+    // ([nPts][nF], transposed on the device)
+    void getFInto(double *out, size_t cap);
+    void getFMembers(double *out, size_t cap);
 
     // trajectory
     void trajectory();
@@ -136,6 +140,9 @@ private:
     void setNpts(int n);
     void upload(DeviceBuffer &b, const std::vector<double> &v, size_t elem);
     std::vector<double> download(const DeviceBuffer &b, size_t n, size_t elem);
+    // n elements of b (elem bytes each) as double into out
+    void downloadInto(const DeviceBuffer &b, size_t n, size_t elem, double *out);
+    DeviceBuffer d_Fm;  // member-major copy of F for getFMembers
     ClodeArgs makeArgs();
     void launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads);
     void updateObserverInfo();

@@ -162,7 +162,6 @@ class FeatureSimulator(Simulator):
                                       observer_min_x_amp, observer_min_imi, observer_neighbourhood_radius,
                                       observer_x_up_thresh, observer_x_down_thresh, observer_dx_up_thresh,
                                       observer_dx_down_thresh, observer_eps_dx)
-        self._device_features: Optional[np.ndarray] = None
         self._num_features: Optional[int] = None
         super().__init__(variables=variables, parameters=parameters, aux=aux, num_noise=num_noise,
                          src_file=src_file, rhs_equation=rhs_equation,
@@ -226,7 +225,6 @@ class FeatureSimulator(Simulator):
             self._integrator.features(initialize_observer)
         else:
             self._integrator.features()
-        self._device_features = None
         self._device_final_state = self._device_dt = self._device_tf = None
         if update_x0:
             self._integrator.shift_x0()
@@ -243,7 +241,6 @@ class FeatureSimulator(Simulator):
         Global spans only."""
         self._integrator.transient_features(list(t_transient), list(t_features))
         self._t_span = tuple(t_features)
-        self._device_features = None
         self._device_final_state = self._device_dt = self._device_tf = None
         self._device_initial_state = None
         if update_x0:
@@ -253,9 +250,10 @@ class FeatureSimulator(Simulator):
         return None
 
     def get_observer_results(self) -> ObserverOutput:
-        if self._device_features is None:
-            self._num_features = self._integrator.get_n_features()
-            self._device_features = np.array(self._integrator.get_f(), dtype=np.float64).reshape(
-                (self._ensemble_size, self._num_features), order="F")
-        return ObserverOutput(self._op, self._device_features, self._num_features, self.variable_names,
+        """The features of the last launch, read from the device on every call
+        (each result owns its array): (members, features) in C order,
+        transposed on the device before the copy."""
+        self._num_features = self._integrator.get_n_features()
+        F = np.asarray(self._integrator.get_f_members())
+        return ObserverOutput(self._op, F, self._num_features, self.variable_names,
                               self._observer_type, self._integrator.get_feature_names(), self._ensemble_shape)

@@ -165,6 +165,11 @@ int clode_get_observer_params(clode_sim *sim, clode_observer_params *op);
 int clode_get_observer_name(clode_sim *sim, char *buf, size_t cap, size_t *needed);
 int clode_get_n_features(clode_sim *sim, int *n);
 int clode_get_f(clode_sim *sim, double *out, size_t n);
+/* F member-major, out[member * nFeatures + feature] (n >= nFeatures * nPts):
+ * the layout the reference's Python layer builds from getF by a host-side
+ * reshape (features.py:467-507), transposed here on the device before the
+ * copy to the host */
+int clode_get_f_members(clode_sim *sim, double *out, size_t n);
 int clode_get_feature_names(clode_sim *sim, char *buf, size_t cap, size_t *needed);
 int clode_get_available_observers(clode_sim *sim, char *buf, size_t cap, size_t *needed);
 
