@@ -129,6 +129,10 @@ constexpr int TK = 32, KPAD = 8;
 template <int T>
 struct Tile;
 template <>
+struct Tile<32> {
+    static constexpr int TM = 32, TN = 32, KC = 128, WB = 1;
+};
+template <>
 struct Tile<64> {
     static constexpr int TM = 64, TN = 64, KC = 128, WB = 2;
 };
@@ -269,8 +273,7 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
         __syncthreads();
         const bool vec = (g.ldt % 8) == 0 && ((uintptr_t)g.YTout & 15) == 0;
 #pragma unroll
-        for (int rr = 0; rr < TN * TM / 8 / 256; ++rr) {
-            const int slot = (int)threadIdx.x + 256 * rr;
+        for (int slot = (int)threadIdx.x; slot < TN * TM / 8; slot += 256) {
             const int nl = slot / (TM / 8), ml = (slot % (TM / 8)) * 8;
             const int n = n0 + nl, m = m0 + ml;
             if (n >= g.N)
@@ -841,6 +844,7 @@ struct Layer {
 
 struct clode_mlp {
     int device = 0;
+    int cus = 256;  // compute units of the device (tile-size choice)
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     Layer L[NLAYER];
@@ -907,11 +911,19 @@ void gemm_tiled(clode_mlp *h, int epi, const GemmArgs &a)
 }
 
 // 128 x 128 tiles once the product has enough rows to keep every CU busy
-// with them (screening chunks); 64 x 64 below that (training minibatches)
+// with them (screening chunks); 64 x 64 below that; 32 x 32 when 64 x 64
+// tiles would leave CUs idle (the training minibatch products: 2048 x 256
+// is 128 tiles of 64 x 64 on 256 CUs).  Every tile size accumulates each
+// output element over the same K chunks in the same order, so the results
+// are identical.
 void gemm(clode_mlp *h, int epi, const GemmArgs &a)
 {
+    const long long splits = a.ksplit > 0 ? (a.K + a.ksplit - 1) / a.ksplit : 1;
+    const long long tiles64 = (long long)((a.M + 63) / 64) * ((a.N + 63) / 64) * splits;
     if (a.ksplit == 0 && a.M >= 32768)
         gemm_tiled<128>(h, epi, a);
+    else if (tiles64 < h->cus)
+        gemm_tiled<32>(h, epi, a);
     else
         gemm_tiled<64>(h, epi, a);
 }
@@ -1171,6 +1183,7 @@ int clode_mlp_create(clode_mlp **out, int device, unsigned long long seed)
     auto *h = new clode_mlp;
     try {
         h->device = device;
+        check(hipDeviceGetAttribute(&h->cus, hipDeviceAttributeMultiprocessorCount, device), "hipDeviceGetAttribute");
         check(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking), "hipStreamCreate");
         check(hipEventCreate(&h->ev0), "hipEventCreate");
         check(hipEventCreate(&h->ev1), "hipEventCreate");

@@ -49,14 +49,15 @@ def test_lactotroph_256x256_sample_bitexact(gpu_available):
     assert np.array_equal(F[idx], ref)
 
 
-def test_lorenz_2p20_fp32c(gpu_available):
+@pytest.mark.parametrize("stepper", ["dopri5", "bs23"])
+def test_lorenz_2p20_fp32c(gpu_available, stepper):
     n = 1 << 20
     rng = np.random.default_rng(5)
     x0 = rng.normal(1.0, 0.1, size=(n, 3))
     rho = np.linspace(0.0, 250.0, n)
     sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lorenz.cl"), variables={"x": 1.0, "y": 1.0, "z": 1.0},
                                  parameters={"sigma": 10.0, "rho": 28.0, "beta": 8 / 3},
-                                 observer=clode.Observer.local_max, stepper=clode.Stepper.dormand_prince,
+                                 observer=clode.Observer.local_max, stepper=clode.Stepper(stepper),
                                  single_precision="fp32c", dt=0.01, dtmax=1.0, abstol=1e-8, reltol=1e-6,
                                  max_steps=10 ** 8, t_span=(0.0, 50.0))
     sim.set_ensemble(variables=x0, parameters={"rho": rho})
@@ -83,7 +84,7 @@ def test_lorenz_2p20_fp32c(gpu_available):
     idx = np.arange(100, 54000, 131)
     refs = {}
     for rtol in (1e-6, 1e-8):
-        o = OracleEnsemble("lorenz", 3, 3, stepper="dopri5", observer="localmax", precision="double")
+        o = OracleEnsemble("lorenz", 3, 3, stepper=stepper, observer="localmax", precision="double")
         o.set_solver_params(0.01, 1.0, rtol * 1e-2, rtol, 10 ** 8, 1000000, 1)
         o.set_observer_params()
         o.set_problem_data(x0[idx].T, np.vstack([10.0 * np.ones(idx.size), rho[idx], np.full(idx.size, 8 / 3)]))
