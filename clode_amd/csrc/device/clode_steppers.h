@@ -32,7 +32,7 @@
 
 #if CLODE_STEPPER == 1 || CLODE_STEPPER == 2 || CLODE_STEPPER == 3 || CLODE_STEPPER == 6
 #define CLODE_FIXED_STEP 1
-#elif CLODE_STEPPER == 4 || CLODE_STEPPER == 5
+#elif CLODE_STEPPER == 4 || CLODE_STEPPER == 5 || CLODE_STEPPER == 7
 #define CLODE_ADAPTIVE_STEP 1
 #else
 #error "unknown CLODE_STEPPER"
@@ -166,6 +166,33 @@ CLODE_DEV bool step_attempt(realtype &ti, realtype xi[], realtype k1[], const re
 #endif
     getRHS(ti, xi, pars, k1, aux, wi);
     return true;
+}
+#endif
+
+#if CLODE_STEPPER == 7
+// Heun-Euler 1(2) (adaptive_he12.clh:9-40, selected by HEUN_EULER in
+// steppers.cl:89-91 but not in the reference's name map): an Euler
+// predictor, the trapezoidal average as the new state, half the slope
+// difference as the error; the slope at the new state is evaluated again.
+#define CLODE_ROOT_N 2  // 1 / (error order 1 + 1): the square root
+#define CLODE_MAX_SHRINK RCONST(0.5)
+#define CLODE_MAX_GROW RCONST(5.0)
+CLODE_DEV realtype do_step(realtype &ti, realtype xi[], realtype k1[], const realtype pars[],
+                           const realtype dt, realtype aux[], realtype err[], const realtype wi[])
+{
+    realtype tNew = ti + dt;
+    realtype newDt = tNew - ti;
+    realtype xtmp[N_VAR], k2[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + newDt * k1[k];
+    getRHS(tNew, xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++)
+        xi[k] = xi[k] + newDt * RCONST(0.5) * (k1[k] + k2[k]);
+    for (int k = 0; k < N_VAR; k++)
+        err[k] = newDt * RCONST(0.5) * (-k1[k] + k2[k]);
+    getRHS(tNew, xi, pars, k1, aux, wi);
+    ti = tNew;
+    return newDt;
 }
 #endif
 
@@ -425,6 +452,33 @@ CLODE_DEV bool step_attempt(double &ti, realtype xi[], realtype xc[], realtype k
 #endif
     getRHS((realtype)ti, xi, pars, k1, aux, wi);
     return true;
+}
+#endif
+
+#if CLODE_STEPPER == 7
+#define CLODE_EXPON 0.5f
+#define CLODE_MAX_SHRINK 0.5
+#define CLODE_MAX_GROW 5.0
+// Heun-Euler 1(2) in float stages; returns the step taken, newxi holds the
+// new state without the compensation term, dx its increment, err the error
+CLODE_DEV double do_step(double &ti, realtype xi[], realtype k1[], const realtype pars[],
+                         const double dt, realtype aux[], realtype err[], realtype dx[], const realtype wi[])
+{
+    double tNew = ti + dt;
+    double newDt = tNew - ti;
+    const realtype h = (realtype)newDt;
+    realtype xtmp[N_VAR], k2[N_VAR];
+    for (int k = 0; k < N_VAR; k++)
+        xtmp[k] = xi[k] + h * k1[k];
+    getRHS((realtype)tNew, xtmp, pars, k2, aux, wi);
+    for (int k = 0; k < N_VAR; k++) {
+        dx[k] = h * 0.5f * (k1[k] + k2[k]);
+        err[k] = h * 0.5f * (-k1[k] + k2[k]);
+        xi[k] = xi[k] + dx[k];
+    }
+    getRHS((realtype)tNew, xi, pars, k1, aux, wi);
+    ti = tNew;
+    return newDt;
 }
 #endif
 

@@ -24,7 +24,8 @@
 //       and fma-based products), with the sign of x for odd integer y; the
 //       float version rounds the double result once.
 //  pown: binary exponentiation (exact operation sequence).
-//  root<N>: x^(1/N), N = 3, 5, by division-free Newton iteration (below).
+//  root<N>: x^(1/N), N = 3, 5, by division-free Newton iteration (below);
+//           N = 2 is the square root.
 //
 // CM_FN is defined by the including translation unit (host or device).
 #ifndef CLODE_CM_MATH_H
@@ -198,7 +199,8 @@ CM_FN double pown(double x, int n)
 }
 
 // x^(1/N) for N = 3 or 5: the adaptive controller's factor (rtol/err)^(1/(p+1))
-// for the error orders p = 2 (bs23) and 4 (dopri5), with the special cases of
+// for the error orders p = 2 (bs23) and 4 (dopri5) (N = 2, p = 1 for he12, is
+// the IEEE square root, which is what pow(x, 0.5) rounds to), with the special cases of
 // pow(x, 1/N): NaN for NaN or x < 0, 0 for 0, inf for inf.  Branch-free and
 // division-free:
 //   x = M 2^(N q) with M = m 2^r, m in [1/2, 1), 0 <= r < N;
@@ -211,7 +213,9 @@ CM_FN double pown(double x, int n)
 template <int N>
 CM_FN double root(double x)
 {
-    static_assert(N == 3 || N == 5, "root<N>: N = 3 or 5");
+    static_assert(N == 2 || N == 3 || N == 5, "root<N>: N = 2, 3 or 5");
+    if (N == 2)  // pow(x, 0.5) is the correctly rounded square root (+0 for -0)
+        return x == 0.0 ? 0.0 : sqrt(x);
     const double inf = __builtin_huge_val();
     double xs = (x > 0.0 && x < inf) ? x : 1.0;
     int e;

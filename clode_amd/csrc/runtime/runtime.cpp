@@ -55,7 +55,10 @@ void hip_check(hipError_t e, const char *what)
 const std::map<std::string, int> &stepper_map()
 {
     static const std::map<std::string, int> m = {
-        {"euler", 1}, {"heun", 2}, {"rk4", 3}, {"bs23", 4}, {"dopri5", 5}, {"seuler", 6}};
+        {"euler", 1}, {"heun", 2}, {"rk4", 3}, {"bs23", 4}, {"dopri5", 5}, {"seuler", 6},
+        // adaptive_he12.clh: compiled by the reference under HEUN_EULER but
+        // missing from its name map (steppers.cl:26-43)
+        {"he12", 7}};
     return m;
 }
 

@@ -27,6 +27,7 @@ class Stepper(Enum):
     bs23 = "bs23"
     dormand_prince = "dopri5"
     stochastic_euler = "seuler"
+    heun_euler = "he12"  # adaptive_he12.clh (not in the reference's name map)
 
 
 def _precision_arg(single_precision: Union[bool, str]) -> Union[bool, str]:

@@ -77,7 +77,7 @@ inline void mark_varying(R &)
 {
 }
 
-enum { ST_EULER = 1, ST_HEUN = 2, ST_RK4 = 3, ST_BS23 = 4, ST_DOPRI5 = 5, ST_SEULER = 6 };
+enum { ST_EULER = 1, ST_HEUN = 2, ST_RK4 = 3, ST_BS23 = 4, ST_DOPRI5 = 5, ST_SEULER = 6, ST_HE12 = 7 };
 enum { OB_NONE = 0, OB_THRESH2 = 1, OB_LOCALMAX = 2, OB_NHOOD2 = 3, OB_BASIC = 4, OB_BASICALL = 5, OB_NHOOD1 = 6 };
 
 template <class R>
@@ -243,6 +243,29 @@ struct Integrator {
         return 0;
     }
 
+    // adaptive_he12.clh:9-40 (Heun-Euler 1(2): an Euler predictor, the
+    // trapezoidal average as the new state, half the slope difference as the
+    // error; the slope at the new state is evaluated again for k1)
+    R he12_step(R *ti, R *xi, R *k1, const R *p, R dt, R *aux, R *err, const R *wi) const
+    {
+        const int N = pb.nVar;
+        R tNew = *ti + dt;
+        R newDt = tNew - *ti;
+        Buf<R> xtmp, k2;
+        for (int k = 0; k < N; k++)
+            xtmp[k] = xi[k] + newDt * k1[k];
+        rhs(tNew, xtmp.data(), p, k2.data(), aux, wi);
+        for (int k = 0; k < N; k++)
+            xi[k] = xi[k] + newDt * OC(0.5) * (k1[k] + k2[k]);
+        for (int k = 0; k < N; k++)
+            err[k] = newDt * OC(0.5) * (-k1[k] + k2[k]);
+        rhs(tNew, xi, p, k2.data(), aux, wi);
+        for (int k = 0; k < N; k++)
+            k1[k] = k2[k];
+        *ti = tNew;
+        return newDt;
+    }
+
     // adaptive_bs23.clh:3-62
     R bs23_step(R *ti, R *xi, R *k1, const R *p, R dt, R *aux, R *err, const R *wi) const
     {
@@ -321,12 +344,13 @@ struct Integrator {
     {
         CLODE_ORACLE_TRIAL_BEGIN();
         const int N = pb.nVar;
-        const bool bs = stepper == ST_BS23;
-        const R maxShrink = bs ? OC(0.5) : OC(0.1);
+        const bool bs = stepper == ST_BS23, he = stepper == ST_HE12;
+        const R maxShrink = (bs || he) ? OC(0.5) : OC(0.1);
         const R maxGrow = OC(5.0);
         // the reference's pow(rtol/err, 1/(p+1)) as the fixed root x^(1/(p+1)),
-        // p = 2 (bs23) or 4 (dopri5), shared with the device (cm_math.h)
-        auto root = [bs](R v) { return bs ? cm::root<3>(v) : cm::root<5>(v); };
+        // p = 1 (he12: the square root), 2 (bs23) or 4 (dopri5), shared with
+        // the device (cm_math.h)
+        auto root = [bs, he](R v) { return he ? cm::root<2>(v) : bs ? cm::root<3>(v) : cm::root<5>(v); };
         R tNew, normErr;
         Buf<R> err, newxi, newk1;
         R newDt = *dt;
@@ -354,7 +378,9 @@ struct Integrator {
                 newxi[j] = xi[j];
                 newk1[j] = k1[j];
             }
-            if (bs)
+            if (he)
+                newDt = he12_step(&tNew, newxi.data(), newk1.data(), p, newDt, aux, err.data(), wi);
+            else if (bs)
                 newDt = bs23_step(&tNew, newxi.data(), newk1.data(), p, newDt, aux, err.data(), wi);
             else
                 newDt = dopri5_step(&tNew, newxi.data(), newk1.data(), p, newDt, aux, err.data(), wi);
@@ -397,7 +423,7 @@ struct Integrator {
     int step(R *ti, R *xi, R *k1, const R *p, const SolverParamsT<R> &sp, R *dt, R t1, R *aux, R *wi,
              RngData<R> &rd) const
     {
-        if (stepper == ST_BS23 || stepper == ST_DOPRI5)
+        if (stepper == ST_BS23 || stepper == ST_DOPRI5 || stepper == ST_HE12)
             return adaptive_stepper(ti, xi, k1, p, sp, dt, t1, aux, wi, rd);
         return fixed_stepper(ti, xi, k1, p, sp, dt, t1, aux, wi, rd);
     }

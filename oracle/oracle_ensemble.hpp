@@ -117,7 +117,7 @@ struct Ensemble {
         m.ti = mt0(i);
         m.dt = dt[i];
         mark_varying(m.ti);
-        if (cfg.stepper == ST_BS23 || cfg.stepper == ST_DOPRI5)
+        if (cfg.stepper == ST_BS23 || cfg.stepper == ST_DOPRI5 || cfg.stepper == ST_HE12)
             mark_varying(m.dt);  // a fixed step size is loop-invariant
         for (int j = 0; j < cfg.nPar; ++j)
             m.p[j] = pars[(size_t)j * nPts + i];

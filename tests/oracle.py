@@ -19,7 +19,7 @@ REPO = Path(__file__).resolve().parent.parent
 sys.path.insert(0, str(REPO / "oracle"))
 import build_oracle  # noqa: E402
 
-STEPPERS = {"euler": 1, "heun": 2, "rk4": 3, "bs23": 4, "dopri5": 5, "seuler": 6}
+STEPPERS = {"euler": 1, "heun": 2, "rk4": 3, "bs23": 4, "dopri5": 5, "seuler": 6, "he12": 7}
 OBSERVERS = {"": 0, "thresh2": 1, "localmax": 2, "nhood2": 3, "basic": 4, "basicall": 5, "nhood1": 6}
 
 _libs: dict[str, ctypes.CDLL] = {}

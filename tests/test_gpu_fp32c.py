@@ -5,6 +5,9 @@ error against the FP64 oracle stated):
   * fixed-step rk4 Van der Pol over 2,000 steps: |x - x_fp64| <= 2e-5 (1 + |x_fp64|);
   * fixed-step Euler on decay over 1e5 steps: the Kahan-compensated state
     error against FP64 is at least 10x smaller than plain single precision;
+  * adaptive Heun-Euler on Van der Pol: the state within 3x FP64's own
+    truncation error (rtol against rtol / 100) of the FP64 oracle on >= 95 %
+    of the members, median below 1e-4 (1 + |x|);
   * dopri5 lactotroph features on a parameter sweep that crosses regime
     boundaries, over members whose event counts agree (>= 90 %): period,
     peaks, min/mean V, mean c median relative error <= 1e-4, 95th percentile
@@ -93,3 +96,33 @@ def test_dopri5_lactotroph_features_vs_fp64(gpu_available):
         rel = np.abs(A[j][ok] - B[j][ok]) / np.abs(B[j][ok])
         assert float(np.median(rel)) <= med, (key, float(np.median(rel)))
         assert float(np.percentile(rel, 95)) <= p95, (key, float(np.percentile(rel, 95)))
+
+
+def test_he12_vdp_state_within_truncation_of_fp64(gpu_available):
+    """Heun-Euler 1(2) in fp32c: the Van der Pol state after t = 20 differs
+    from FP64 at the same tolerance by no more than 3x FP64's own truncation
+    error (FP64 at rtol against rtol / 100) plus 1e-5 (1 + |x|), on at least
+    95 % of the members, with a median difference below 1e-4 (1 + |x|); the
+    two take different step sequences, so their difference is of the
+    method's truncation size, not of float rounding."""
+    n = 129
+    mu = np.linspace(0.1, 3.0, n)
+    x0 = np.vstack([2.0 * np.ones(n), np.zeros(n)])
+    sp = (0.01, 1.0, 1e-7, 1e-5, 10 ** 8, 10, 1)
+    c = Pair("van_der_pol", "he12", "", "fp32c", kind="base", sp=sp, oracle=False)
+    c.set_problem_data(x0, mu[None, :])
+    c.set_tspan(0.0, 20.0)
+    c.transient()
+    refs = []
+    for scale in (1.0, 1e-2):
+        r = Pair("van_der_pol", "he12", "", "double", kind="base",
+                 sp=(sp[0], sp[1], sp[2] * scale, sp[3] * scale) + sp[4:])
+        r.set_problem_data(x0, mu[None, :])
+        r.set_tspan(0.0, 20.0)
+        r.orc.transient()
+        refs.append(r.orc.get("xf"))
+    a = np.array(c.gpu.get_xf())
+    d, t = np.abs(a - refs[0]), np.abs(refs[0] - refs[1])
+    within = d <= 3 * t + 1e-5 * (1 + np.abs(refs[0]))
+    assert np.mean(within) >= 0.95, (float(np.mean(within)), float(np.max(d)))
+    assert float(np.median(d / (1 + np.abs(refs[0])))) < 1e-4

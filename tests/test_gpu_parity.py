@@ -55,6 +55,9 @@ for prec in ("double", "single"):
     CASES.append(("van_der_pol", "rk4", "basicall", prec, 97, (0.0, 60.0), FIXED))
     CASES.append(("lorenz", "bs23", "basic", prec, 130, (0.0, 40.0), ADAPT))
     CASES.append(("lactotroph", "dopri5", "nhood1", prec, 70, (0.0, 1500.0), ADAPT))
+    # Heun-Euler 1(2) (adaptive_he12.clh), a low-order method: looser tolerance
+    CASES.append(("van_der_pol", "he12", "thresh2", prec, 97, (0.0, 60.0), (0.01, 1.0, 1e-6, 1e-4, 10000000, 1000000, 1)))
+    CASES.append(("lorenz", "he12", "localmax", prec, 130, (0.0, 20.0), (0.01, 1.0, 1e-6, 1e-4, 10000000, 1000000, 1)))
     CASES.append(("lactotroph_noise", "seuler", "basicall", prec, 65, (0.0, 400.0),
                   (0.05, 1.0, 1e-6, 1e-3, 1000000, 1000000, 1)))
 

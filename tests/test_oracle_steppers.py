@@ -59,7 +59,7 @@ def test_rk4_harmonic_closed_form():
     assert abs(x - math.cos(tf)) < 1e-9 and abs(y + math.sin(tf)) < 1e-9
 
 
-@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5", "he12"])
 def test_adaptive_tolerance_proportionality(stepper):
     errs = []
     for rtol in (1e-4, 1e-6, 1e-8):
@@ -116,7 +116,7 @@ def first_adaptive_step(stepper, lam, t0, t1, max_steps=1):
     return o.get("xf")[0], o.get("tf")[0], o.get("dt")[0]
 
 
-@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5", "he12"])
 def test_controller_halves_past_one_hundred_trials(stepper):
     """x' = -1e31 x from t = 0 with dt = 1: the first trial fails by ~1e31,
     so the step shrinks to 0.1 (the maximal first shrink: 0.5 for bs23, 0.1
@@ -126,7 +126,7 @@ def test_controller_halves_past_one_hundred_trials(stepper):
     h = s 2^-k (exactly: t = 0 + h is exact) and x(h) = exp(-lambda h).  A
     bound on trials of 100 would instead fail the step at hmin (t stays 0)."""
     x, tf, dt = first_adaptive_step(stepper, 1e31, 0.0, 1.0)
-    shrink = 0.5 if stepper == "bs23" else 0.1
+    shrink = 0.1 if stepper == "dopri5" else 0.5
     assert tf > 0.0
     k = math.log2(shrink / tf)
     assert k == round(k) and k > 100, k
@@ -135,7 +135,7 @@ def test_controller_halves_past_one_hundred_trials(stepper):
     assert 1e31 * tf < 1.0  # the accepted step resolves the decay
 
 
-@pytest.mark.parametrize("stepper", ["bs23", "dopri5"])
+@pytest.mark.parametrize("stepper", ["bs23", "dopri5", "he12"])
 def test_controller_fails_a_stalled_trial_at_hmin(stepper):
     """t0 = 1 - 13 2^-53, just below a power of two: hmin = 16 ulp = 2^-49,
     but t0 + hmin rounds (to even) to 1 + 2^-51, a step of 17 ulp > hmin.  A
@@ -147,3 +147,34 @@ def test_controller_fails_a_stalled_trial_at_hmin(stepper):
     assert (t0 + 2.0 ** -49) - t0 == 17 * 2.0 ** -53
     x, tf, dt = first_adaptive_step(stepper, 1e40, t0, 2.0, max_steps=3)
     assert tf == t0 and x == 1.0 and dt == 2.0 ** -49
+
+
+def test_he12_step_closed_form():
+    """Heun-Euler 1(2) on x' = -lambda x from x = 1 with step h: the Euler
+    predictor 1 - lambda h, the trapezoidal state 1 - lambda h + (lambda h)^2 / 2,
+    the error estimate (lambda h)^2 / 2; the next step is h times
+    min(5, 0.8 sqrt(rtol / err)) (error order 1: the square root), and a
+    rejected first trial shrinks by at most 0.5 (adaptive_he12.clh)."""
+    lam, h, rtol = 1.0, 0.1, 0.1
+    o = OracleEnsemble("decay", 1, 1, stepper="he12", kind="base", threads=1)
+    o.set_solver_params(h, 1.0, 1e-12, rtol, 1, 1000, 1)
+    o.set_problem_data(np.ones((1, 1)), np.full((1, 1), lam))
+    o.set_tspan((0.0, 10.0))
+    o.transient()
+    x, tf, dt = o.get("xf")[0], o.get("tf")[0], o.get("dt")[0]
+    err = (lam * h) ** 2 / 2
+    assert tf == h
+    assert x == pytest.approx(1 - lam * h + (lam * h) ** 2 / 2, rel=1e-15)
+    assert dt == pytest.approx(h * 0.8 * math.sqrt(rtol / err), rel=1e-15)
+    # lambda h = 10: error 50 > rtol, the first shrink is max(0.5, 0.8 sqrt(2e-5)) = 0.5
+    o = OracleEnsemble("decay", 1, 1, stepper="he12", kind="base", threads=1)
+    o.set_solver_params(h, 1.0, 1e-12, 1e-3, 1, 1000, 1)
+    o.set_problem_data(np.ones((1, 1)), np.full((1, 1), 100.0))
+    o.set_tspan((0.0, 10.0))
+    o.transient()
+    tf = o.get("tf")[0]
+    # rejected at 0.1, then 0.05, then halved until 100 h (1 - ...) meets rtol
+    k = math.log2(0.05 / tf)
+    assert k == round(k) and tf <= 0.05
+    lh = 100.0 * tf
+    assert lh * lh / 2 <= 1e-3 < (2 * lh) ** 2 / 2
