@@ -515,6 +515,7 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
     graph_s = time.perf_counter() - t0
     steps = sum(int(s.get_step_counts().sum()) for s in sims)
     F_graph = [s.get_observer_results().to_ndarray() for s in sims]
+    roof = c4_round_roofline(batch, sims)
     # the same sequence, system by system, one-member-per-lane grid launches,
     # over the first grid_rounds rounds (compared per round)
     rng.bit_generator.state = state
@@ -533,7 +534,35 @@ def mixed_c4(n_each=1 << 18, rounds=20, seed=7, grid_rounds=3):
             "graph_sweep_s": graph_s, "s_per_round": graph_s / rounds, "member_steps": steps,
             "member_steps_per_s": steps / graph_s, "grid_rounds": grid_rounds, "grid_s_per_round": grid_s / grid_rounds,
             "speedup_vs_grid_per_round": (grid_s / grid_rounds) / (graph_s / rounds),
-            "identical_features": same}
+            "identical_features": same, "round_roofline": roof}
+
+
+def c4_round_roofline(batch, sims):
+    """One more round of the mixed sweep, transient and features as separate
+    batched launches, priced with the generated operation table: each
+    system's device-counted steps and trials per phase times its per-step and
+    per-rejected-trial FLOPs, over the two launches' device time.  (The
+    observers continue, so the features launch has no warm-up pass.)"""
+    from clode_amd.perfmodel import PEAK_TFLOPS, launch_flops
+
+    names = ("mixed_vdp_dopri5_thresh2", "mixed_lorenz_dopri5_localmax", "mixed_chay_keizer_dopri5_thresh2",
+             "mixed_ou_seuler_basicall")
+    flops, ms, per = 0.0, 0.0, {}
+    for phase, run in (("transient", lambda: batch.transient(update_x0=False)),
+                       ("features", lambda: batch.features(update_x0=False))):
+        for s in sims:
+            s.reset_step_counts()
+        run()
+        ms += batch.last_kernel_ms()
+        batch.shift_x0()
+        for w, s in zip(names, sims):
+            st, tr = int(s.get_step_counts().sum()), int(s.get_trial_counts().sum())
+            f = launch_flops(w, phase, st, tr)
+            flops += f
+            per.setdefault(w, {})[phase] = {"member_steps": st, "trials": tr, "flops": f}
+    tf = flops / (ms / 1e3) / 1e12
+    return {"kernel_ms": ms, "algorithmic_flops": flops, "algorithmic_tflops": tf,
+            "frac_of_fp32_peak": tf / PEAK_TFLOPS["f32"], "systems": per}
 
 
 def surrogate_c5(precision, n=2048, held_out=False):
