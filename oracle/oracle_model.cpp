@@ -560,11 +560,13 @@ void oracle_cm_eval(int fn, int prec, const double *x, const double *y, double *
         if (prec == 1) {
             float a = (float)x[k], b = y ? (float)y[k] : 0.f;
             out[k] = fn == 0 ? cm::exp(a) : fn == 1 ? cm::log(a) : fn == 2 ? cm::pow(a, b) : fn == 3 ? cm::pown(a, (int)b)
-                   : fn == 4 ? cm::root<3>(a) : fn == 5 ? cm::root<5>(a) : cm::min_step(a, b);
+                   : fn == 4 ? cm::root<3>(a) : fn == 5 ? cm::root<5>(a) : fn == 7 ? cm::root<2>(a)
+                   : cm::min_step(a, b);
         } else {
             double a = x[k], b = y ? y[k] : 0.0;
             out[k] = fn == 0 ? cm::exp(a) : fn == 1 ? cm::log(a) : fn == 2 ? cm::pow(a, b) : fn == 3 ? cm::pown(a, (int)b)
-                   : fn == 4 ? cm::root<3>(a) : fn == 5 ? cm::root<5>(a) : cm::min_step(a, b);
+                   : fn == 4 ? cm::root<3>(a) : fn == 5 ? cm::root<5>(a) : fn == 7 ? cm::root<2>(a)
+                   : cm::min_step(a, b);
         }
     }
 }

@@ -232,7 +232,7 @@ def cm_eval(fn: str, x, y=None, precision: str = "double", model: str = "decay")
     x = np.ascontiguousarray(x, dtype=np.float64)
     yy = None if y is None else np.ascontiguousarray(y, dtype=np.float64)
     out = np.zeros_like(x)
-    lib.oracle_cm_eval({"exp": 0, "log": 1, "pow": 2, "pown": 3, "root3": 4, "root5": 5, "min_step": 6}[fn],
+    lib.oracle_cm_eval({"exp": 0, "log": 1, "pow": 2, "pown": 3, "root3": 4, "root5": 5, "min_step": 6, "root2": 7}[fn],
                        1 if precision == "single" else 2,
                        _dptr(x), None if yy is None else _dptr(yy), _dptr(out), x.size)
     return out

@@ -144,3 +144,20 @@ def test_min_step_is_sixteen_spacings(precision):
     # the fast branch covers the usual case: a positive time short of t1
     usual = (t > fi.tiny) & (t < toward) & np.isfinite(t)
     assert usual.mean() > 0.5
+
+
+@pytest.mark.parametrize("precision", ["double", "single"])
+def test_root2_is_the_square_root(precision):
+    """cm::root<2>, the he12 controller's (rtol/err)^(1/2): pow(x, 0.5) with
+    its exponent exact is the correctly rounded square root, so the root is
+    numpy's sqrt bit for bit (+0 for -0, NaN below zero, inf for inf)."""
+    rng = np.random.default_rng(9)
+    hi = 300 if precision == "double" else 37
+    x = np.concatenate([10 ** rng.uniform(-hi, hi, 100000), rng.uniform(0, 4, 10000), [0.0, np.inf]])
+    if precision == "single":
+        x = x.astype(np.float32).astype(np.float64)
+    got = cm_eval("root2", x, precision=precision)
+    want = np.sqrt(x) if precision == "double" else np.sqrt(x.astype(np.float32)).astype(np.float64)
+    assert np.array_equal(got, want)
+    sp = cm_eval("root2", np.array([-0.0, -1.0, np.nan]), precision=precision)
+    assert sp[0] == 0.0 and not np.signbit(sp[0]) and np.isnan(sp[1]) and np.isnan(sp[2])
