@@ -28,7 +28,7 @@ is FP64 at rtol against FP64 at rtol / 100 on the same members.  Stated bounds:
   the extreme slopes to 5e-5 (the float normals of the largest draws carry
   ~1e-5 relative error from the polar method's 2 U - 1), xf to
   1e-5 (1 + |x|) and the time mean to 1e-5 of the range.
-  Lorenz uses rho in [0, 13) (no chaos) and short spans (median
+  Lorenz uses rho in [0, 13) (no chaotic transient) and short spans (median
   5), before the damped oscillation sinks below float resolution, where
   fp32c's localmax counts rounding noise as maxima (documented limitation).
 * Lorenz 2^20, bs23 and dopri5: see tests/test_gpu_fullsize.py.
