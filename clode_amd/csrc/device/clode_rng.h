@@ -5,7 +5,7 @@
 // Follows clode/cpp/clODE_random.cl:93-183: the 64-bit output s0+s1 is
 // converted to realtype and scaled by 2^-64 (so in single precision the draw
 // can round up to 1.0f, exactly as the reference does), the polar loop rejects
-// w >= 1, and the cache flag is reset at every kernel launch by the driver
+// w >= 1, and the cache flag is reset at every kernel launch by the kernel
 // (clode/cpp/transient.cl:42).  The 64-bit stream is therefore identical bit
 // for bit to the reference's for the same seed words.
 #ifndef CLODE_RNG_H
