@@ -1,0 +1,68 @@
+"""GPU parity on randomly drawn configurations, bit for bit against the CPU
+oracle: a fixed list of seeds draws the model, stepper, observer, precision,
+ensemble size, solver settings and time span of each case.  Besides widening
+the fixed parity matrix, the draws put the time spans where the controller's
+minimum step takes its other branches (negative times, spans that cross
+zero, times just below a power of two, large offsets), with per-member spans
+on some cases, and the observers' event lists on.
+"""
+import numpy as np
+import pytest
+
+from helpers import Pair
+
+pytestmark = pytest.mark.gpu
+
+MODELS = {
+    "van_der_pol": (2, 1, lambda rng, n: (np.vstack([rng.uniform(1.0, 2.5, n), rng.uniform(-1, 1, n)]),
+                                         rng.uniform(0.1, 4.0, n)[None, :]), 40.0),
+    "lorenz": (3, 3, lambda rng, n: (rng.normal(1.0, 0.1, (3, n)),
+                                     np.vstack([10.0 * np.ones(n), rng.uniform(0, 60, n), np.full(n, 8 / 3)])), 10.0),
+    "lactotroph": (4, 4, lambda rng, n: (np.tile(np.array([[-60.0], [0.0], [0.0], [0.1]]), (1, n)),
+                                         np.vstack([1.5 * np.ones(n), 3.0 * np.ones(n), rng.uniform(0, 2, n),
+                                                    rng.uniform(0, 10, n)])), 600.0),
+}
+ADAPTIVE = ("bs23", "dopri5", "he12")
+FIXED = ("euler", "heun", "rk4")
+
+
+def draw(seed):
+    rng = np.random.default_rng(seed)
+    model = ["van_der_pol", "lorenz", "lactotroph"][seed % 3]
+    stepper = (ADAPTIVE + FIXED)[rng.integers(6)]
+    if stepper in FIXED and model == "lactotroph":
+        stepper = "dopri5"  # stiff spikes: a fixed step of this size diverges
+    observer = ["thresh2", "localmax", "nhood2", "basicall"][rng.integers(4)]
+    prec = ["double", "single"][rng.integers(2)]
+    n = int(rng.integers(33, 150))
+    _, _, data, length = MODELS[model]
+    start = [0.0, -0.5 * length, 1.0 - 2.0 ** -20, 1000.0, -3.0, 0.25][rng.integers(6)]
+    # a tolerance the precision can meet (float carries about 7 digits)
+    rtol = float(10 ** (rng.uniform(-8, -4) if prec == "double" else rng.uniform(-5, -3)))
+    dt = 0.01 if stepper in FIXED else float(10 ** rng.uniform(-3, -1))
+    # Before t = 0 the reference's minimum step 16 |fabs(nextafter(t, 1.1 t1)) - t|
+    # is about 2 |t|, above dtmax: every adaptive step then fails at hmin
+    # without advancing, until max_steps ends the member (reproduced on both
+    # sides); a smaller step budget keeps those cases short.
+    max_steps = 20000 if (start < 0 and stepper in ADAPTIVE) else 200000
+    sp = (dt, 1.0, rtol * 1e-2, rtol, max_steps, 10 ** 6, 1)
+    per_member = bool(rng.integers(2))
+    return model, stepper, observer, prec, n, data(rng, n), (start, start + length), sp, per_member, rng
+
+
+@pytest.mark.parametrize("seed", list(range(24)))
+def test_random_configuration_bitexact(gpu_available, seed):
+    model, stepper, observer, prec, n, (x0, pars), tspan, sp, per_member, rng = draw(seed)
+    print(f"seed {seed}: {model} {stepper} {observer} {prec} n={n} tspan={tspan} sp={sp} "
+          f"per_member={per_member}", flush=True)
+    p = Pair(model, stepper, observer, prec, obs=dict(max_event_timestamps=int(rng.integers(0, 4))), sp=sp)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(*tspan)
+    if per_member:
+        t0 = tspan[0] + rng.uniform(-1.0, 1.0, n)
+        p.set_member_tspan(np.vstack([t0, t0 + (tspan[1] - tspan[0]) * rng.uniform(0.2, 1.0, n)]))
+    p.transient()
+    p.assert_bitexact(("xf", "dt", "tf"))
+    p.shift_x0()
+    p.features()
+    p.assert_bitexact(("F", "xf", "dt", "tf"))
