@@ -14,8 +14,8 @@
 //
 // Methods:
 //  exp: Cody-Waite argument reduction x = k ln2 + r, |r| <= ln2/2, followed by
-//       the Taylor polynomial of e^r (degree 13 for double, 7 for float) in
-//       Horner form, scaled exactly by 2^k with ldexp.
+//       e^r = 1 + r + r^2 Q(r) with a fitted polynomial Q (degree 9 for double,
+//       4 for float) in Horner form, scaled exactly by 2^k with ldexp.
 //  log: x = m 2^e with m in [sqrt(1/2), sqrt(2)), f = m - 1, s = f/(2+f);
 //       log(1+f) = f - (f^2/2 - s (f^2/2 + R(s^2))) where R is the series
 //       2z/3 + 2z^2/5 + ... of 2 atanh(s)/s - 2 (11 terms double, 4 float).
@@ -24,7 +24,7 @@
 //       and fma-based products), with the sign of x for odd integer y; the
 //       float version rounds the double result once.
 //  pown: binary exponentiation (exact operation sequence).
-//  root<N>: x^(1/N), N = 3, 5, by division-free Newton iteration (below);
+//  root<N>: x^(1/N), N = 3, 5, from a polynomial guess and Newton steps (below);
 //           N = 2 is the square root.
 //
 // CM_FN is defined by the including translation unit (host or device).
@@ -40,9 +40,11 @@ namespace cm {
 // ---------------------------------------------------------------- double --
 CM_FN double exp(double x)
 {
-    // Branch-free: the reduction runs on x clamped to the finite-result range
-    // (NaN clamps to -746) and the IEEE special cases are selected at the end,
-    // so the compiler can interleave independent calls.
+    // Branch-free: the reduction runs on x clamped to [-746, 710] (NaN clamps
+    // to -746) and NaN is selected at the end.  Overflow and underflow need no
+    // select: above ln(DBL_MAX) the reduced r is positive, so 2^k p with
+    // k = 1024 and p > 1 overflows to inf, and below -1075 ln 2 it is
+    // negative, so 2^-1075 p with p < 1 rounds to 0.
     const double log2e = 0x1.71547652b82fep+0;
     const double ln2hi = 0x1.62e42fefa39efp-1;
     const double ln2lo = 0x1.abc9e3b39803fp-56;
@@ -50,23 +52,22 @@ CM_FN double exp(double x)
     double k = rint(xc * log2e);
     double r = fma(-k, ln2hi, xc);
     r = fma(-k, ln2lo, r);
-    double p = 0x1.6124613a86d09p-33;
-    p = fma(p, r, 0x1.1eed8eff8d898p-29);
-    p = fma(p, r, 0x1.ae64567f544e4p-26);
-    p = fma(p, r, 0x1.27e4fb7789f5cp-22);
-    p = fma(p, r, 0x1.71de3a556c734p-19);
-    p = fma(p, r, 0x1.a01a01a01a01ap-16);
-    p = fma(p, r, 0x1.a01a01a01a01ap-13);
-    p = fma(p, r, 0x1.6c16c16c16c17p-10);
-    p = fma(p, r, 0x1.1111111111111p-7);
-    p = fma(p, r, 0x1.5555555555555p-5);
-    p = fma(p, r, 0x1.5555555555555p-3);
-    p = fma(p, r, 0x1.0000000000000p-1);
+    // exp(r) = 1 + r + r^2 Q(r) on |r| <= ln2/2: Q of degree 9 interpolating
+    // (e^r - 1 - r)/r^2 at the Chebyshev nodes (|error| 1.1e-16, relative
+    // error of exp 1.7e-17)
+    double p = 0x1.af38a57235af4p-26;
+    p = fma(p, r, 0x1.28917e8165204p-22);
+    p = fma(p, r, 0x1.71de0db19876ap-19);
+    p = fma(p, r, 0x1.a019b912dc201p-16);
+    p = fma(p, r, 0x1.a01a01a7c2483p-13);
+    p = fma(p, r, 0x1.6c16c1788a61fp-10);
+    p = fma(p, r, 0x1.11111111109b9p-7);
+    p = fma(p, r, 0x1.5555555553d67p-5);
+    p = fma(p, r, 0x1.5555555555556p-3);
+    p = fma(p, r, 0x1.0000000000001p-1);
     p = fma(p, r, 1.0);
     p = fma(p, r, 1.0);
     double y = ldexp(p, (int)k);
-    y = (x > 709.782712893384) ? __builtin_huge_val() : y;
-    y = (x < -745.1332191019412) ? 0.0 : y;
     return (x == x) ? y : x;
 }
 
@@ -201,13 +202,15 @@ CM_FN double pown(double x, int n)
 // x^(1/N) for N = 3 or 5: the adaptive controller's factor (rtol/err)^(1/(p+1))
 // for the error orders p = 2 (bs23) and 4 (dopri5) (N = 2, p = 1 for he12, is
 // the IEEE square root, which is what pow(x, 0.5) rounds to), with the special cases of
-// pow(x, 1/N): NaN for NaN or x < 0, 0 for 0, inf for inf.  Branch-free and
+// pow(x, 1/N): NaN for NaN or x < 0, +0 for +-0, inf for inf.  Branch-free and
 // division-free:
 //   x = M 2^(N q) with M = m 2^r, m in [1/2, 1), 0 <= r < N;
-//   w ~ M^(-1/N) from a cubic in m and the table 2^(-r/N), then two Newton
-//   steps w <- w + w (1 - M w^N) / N (quadratic convergence from 3e-4);
+//   w ~ M^(-1/N) as P(m) S(r): P the minimax quintic of m^(-1/N) on [1/2, 1)
+//   (relative error 2.7e-6 for N = 5, 5.7e-6 for N = 3) and S the polynomial
+//   of degree N-1 through the N points (r, 2^(-r/N)); one Newton step
+//   w <- w + w (1 - M w^N) / N (relative error 3 eps^2 or 2 eps^2);
 //   y = M w^(N-1) and one Newton step on y^N = M, scaled by 2^q.
-// Within 2 ulp of the exact root (checked in the test-suite against
+// Within 1 ulp of the exact root (checked in the test-suite against
 // long-double references); it differs from pow(x, RN(1/N)) by the exponent's
 // own rounding, |log x| 2^-55 relative.
 template <int N>
@@ -217,7 +220,8 @@ CM_FN double root(double x)
     if (N == 2)  // pow(x, 0.5) is the correctly rounded square root (+0 for -0)
         return x == 0.0 ? 0.0 : sqrt(x);
     const double inf = __builtin_huge_val();
-    double xs = (x > 0.0 && x < inf) ? x : 1.0;
+    const bool finitePos = x > 0.0 && x < inf;
+    double xs = finitePos ? x : 1.0;
     int e;
     double m = frexp(xs, &e);  // xs = m 2^e, m in [0.5, 1), e in [-1073, 1024]
     // floor(e / N) and e mod N without a sign test: 1080 is a multiple of 3
@@ -226,26 +230,22 @@ CM_FN double root(double x)
     const int q = (int)qb - 1080 / N;
     const int r = (int)(eb - N * qb);  // 0 .. N-1
     double M = ldexp(m, r);
-    // w0 ~ m^(-1/N) on [1/2, 1) (least-squares cubic, within 3e-4 relative),
-    // times 2^(-r/N), picked by selects (no branches)
-    double w, sc = 1.0;
+    const double rd = (double)r;
+    double w, sc;
     if (N == 3) {
-        w = fma(fma(fma(-0x1.f8703efde20eap-2, m, 0x1.93be6ca766c32p+0), m, -0x1.02f68e33ca1c6p+1), m,
-                0x1.f03c9f32afbc9p+0);
-        sc = (r == 1) ? 0x1.965fea53d6e3dp-1 : sc;  // 2^(-1/3)
-        sc = (r == 2) ? 0x1.428a2f98d728bp-1 : sc;  // 2^(-2/3)
+        w = fma(fma(fma(fma(fma(-0x1.5f50df22b3948p-1, m, 0x1.92b6d1f866c72p+1), m, -0x1.81aceb53d20cdp+2), m,
+                        0x1.912b993c1c0b2p+2), m, -0x1.f91b483312656p+1), m, 0x1.1f3b21f958b0dp+1);
+        sc = fma(fma(0x1.5ca5af129611ep-6, rd, -0x1.d2150c92f7332p-3), rd, 1.0);  // 2^(-r/3)
     } else {
-        w = fma(fma(fma(-0x1.ee2841a748840p-3, m, 0x1.91ce75060d2cfp-1), m, -0x1.0d5cc4c4abae0p+0), m,
-                0x1.82339d4c4e107p+0);
-        sc = (r == 1) ? 0x1.bdb8cdadbe120p-1 : sc;  // 2^(-1/5)
-        sc = (r == 2) ? 0x1.8406003b2ae5cp-1 : sc;  // 2^(-2/5)
-        sc = (r == 3) ? 0x1.51cb453b9536cp-1 : sc;  // 2^(-3/5)
-        sc = (r == 4) ? 0x1.2611186bae674p-1 : sc;  // 2^(-4/5)
+        w = fma(fma(fma(fma(fma(-0x1.3fec119f23373p-2, m, 0x1.706cb0760b0d0p+0), m, -0x1.63bf8a58843cbp+1), m,
+                        0x1.784027ecd4f80p+1), m, -0x1.ef99a9737ac22p+0), m, 0x1.a62695914b34dp+0);
+        sc = fma(fma(fma(fma(0x1.88970ec6f3ac2p-17, rd, -0x1.c4b4e3b1a9127p-12), rd, 0x1.3a4e88f4a3e7ap-7), rd,
+                     -0x1.1be579c2b43dbp-3), rd, 1.0);  // 2^(-r/5)
     }
     w *= sc;
-    // two Newton steps for w = M^(-1/N): relative error 3e-4 -> 5e-7 -> 2e-12
+    // one Newton step for w = M^(-1/N): relative error 6e-6 -> 7e-11
     const double rN = 1.0 / N;  // folded: RN(1/N)
-    for (int it = 0; it < 2; ++it) {
+    {
         double w2 = w * w;
         double wN = (N == 3) ? w2 * w : (w2 * w2) * w;
         double t = fma(-M, wN, 1.0);
@@ -258,9 +258,9 @@ CM_FN double root(double x)
     double yN = (N == 3) ? y2 * y : (y2 * y2) * y;
     y = fma(-(yN - M) * wN1, rN, y);
     y = ldexp(y, q);
-    y = (x == 0.0) ? 0.0 : y;
-    y = (x == inf) ? inf : y;
-    return (x < 0.0 || !(x == x)) ? __builtin_nan("") : y;
+    // +-0 -> +0 and inf -> inf (x + 0 is +0 for -0); NaN and x < 0 -> NaN
+    double special = (x >= 0.0) ? x + 0.0 : __builtin_nan("");
+    return finitePos ? y : special;
 }
 
 // float: the double root, rounded once
@@ -301,7 +301,8 @@ CM_FN float min_step(float t, float toward)
 // ----------------------------------------------------------------- float --
 CM_FN float exp(float x)
 {
-    // branch-free, as the double version
+    // branch-free, as the double version (overflow above ln(FLT_MAX) with
+    // k = 128, underflow to 0 below -150 ln 2)
     const float log2e = 0x1.715476p+0f;
     const float ln2hi = 0x1.62e430p-1f;
     const float ln2lo = -0x1.05c610p-29f;
@@ -309,17 +310,16 @@ CM_FN float exp(float x)
     float k = rintf(xc * log2e);
     float r = fmaf(-k, ln2hi, xc);
     r = fmaf(-k, ln2lo, r);
-    float p = 0x1.a01a02p-13f;
-    p = fmaf(p, r, 0x1.6c16c2p-10f);
-    p = fmaf(p, r, 0x1.111112p-7f);
-    p = fmaf(p, r, 0x1.555556p-5f);
-    p = fmaf(p, r, 0x1.555556p-3f);
+    // exp(r) = 1 + r + r^2 Q(r): Q of degree 4, minimax in relative error
+    // (relative error of exp 9e-9)
+    float p = 0x1.6c69f2p-10f;
+    p = fmaf(p, r, 0x1.120b1cp-7f);
+    p = fmaf(p, r, 0x1.5555bap-5f);
+    p = fmaf(p, r, 0x1.5554dep-3f);
     p = fmaf(p, r, 0x1.000000p-1f);
     p = fmaf(p, r, 1.0f);
     p = fmaf(p, r, 1.0f);
     float y = ldexpf(p, (int)k);
-    y = (x > 88.72283935546875f) ? __builtin_huge_valf() : y;
-    y = (x < -103.97208404541015625f) ? 0.0f : y;
     return (x == x) ? y : x;
 }
 
