@@ -8,7 +8,8 @@
 //                stage algebra; time, step size, error norm, observer
 //                statistics and continuation state in double; state updates
 //                Kahan-compensated.  Uses the platform's single-precision
-//                math library (exp/log/pow) for speed.
+//                math library (exp/log/pow) and division for speed, with
+//                float denormals flushed to zero.
 #ifndef CLODE_PRELUDE_H
 #define CLODE_PRELUDE_H
 

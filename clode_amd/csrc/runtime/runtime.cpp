@@ -361,9 +361,13 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
                                   // packed FP32 (v_pk_*_f32) is disabled for all device code
                                   "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
                                   "-Wno-unused-parameter", "-Wno-unused-variable"};
-    if (cfg.prec == PREC_FP32C)
+    if (cfg.prec == PREC_FP32C) {
         o.push_back("-fno-hip-fp32-correctly-rounded-divide-sqrt");
-    else
+        // float denormals flush to zero (double ones are kept): the float
+        // division and exponential then need no denormal scaling or mode
+        // switches
+        o.push_back("-fgpu-flush-denormals-to-zero");
+    } else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
     // code-generation tuning: CLODE_HIPRTC_OPTIONS holds extra whitespace-
     // separated compiler options (part of the program cache key)
