@@ -9,7 +9,8 @@
 //                statistics and continuation state in double; state updates
 //                Kahan-compensated.  Uses the platform's single-precision
 //                math library (exp/log/pow) and division for speed, with
-//                float denormals flushed to zero.
+//                float denormals flushed to zero and divisions allowed to
+//                multiply by a reciprocal.
 #ifndef CLODE_PRELUDE_H
 #define CLODE_PRELUDE_H
 

@@ -367,6 +367,9 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
         // division and exponential then need no denormal scaling or mode
         // switches
         o.push_back("-fgpu-flush-denormals-to-zero");
+        // every division may multiply by the reciprocal (the hardware one
+        // in float, a Newton-refined one in double)
+        o.push_back("-freciprocal-math");
     } else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
     // code-generation tuning: CLODE_HIPRTC_OPTIONS holds extra whitespace-
