@@ -9,8 +9,9 @@
 //                statistics and continuation state in double; state updates
 //                Kahan-compensated.  Uses the platform's single-precision
 //                math library (exp/log/pow) and division for speed, with
-//                float denormals flushed to zero and divisions allowed to
-//                multiply by a reciprocal.
+//                float denormals flushed to zero, divisions allowed to
+//                multiply by a reciprocal and math functions allowed to use
+//                the hardware approximations.
 #ifndef CLODE_PRELUDE_H
 #define CLODE_PRELUDE_H
 
