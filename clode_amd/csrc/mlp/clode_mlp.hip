@@ -447,29 +447,28 @@ __global__ void __launch_bounds__(256) mlp_layer_grads(const float *part, int sp
         db[c] = red[0];
 }
 
-// layer table for the Adam kernel: weights of layer l occupy
-// [woff, woff + in * out) as Wt[out][in]; the transposed BF16 copy keeps
-// W[in][out] at the same offsets
+// layer table for the Adam kernels: weights of layer l occupy
+// [woff, woff + in * out) as Wt[out][in], its biases the out entries after
+// them; the transposed BF16 copy keeps W[in][out] at the same offsets
 struct LayerTable {
     long long woff[NLAYER];
     int in[NLAYER], out[NLAYER];
 };
 
-// Adam (Kingma & Ba 2015) on the FP32 master parameters, refreshing both BF16
-// copies of the weights; frozen entries (mask 0: head padding) stay put
-__global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float *m1, float *m2, const uint8_t *mask,
-                                                bf16_t *pbf, bf16_t *pbfT, const LayerTable lt, long long n, float lr,
-                                                float b1, float b2, float eps, float c1, float c2)
+struct AdamStep {
+    float lr, b1, b2, eps, c1, c2;
+};
+
+// Adam (Kingma & Ba 2015) on one FP32 master parameter, refreshing both BF16
+// copies of the weights
+__device__ __forceinline__ void adam_update(long long i, float gi, float *p, float *m1, float *m2, bf16_t *pbf,
+                                            bf16_t *pbfT, const LayerTable &lt, const AdamStep &as)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !mask[i])
-        return;
-    const float gi = g[i];
-    const float a = b1 * m1[i] + (1.f - b1) * gi;
-    const float b = b2 * m2[i] + (1.f - b2) * gi * gi;
+    const float a = as.b1 * m1[i] + (1.f - as.b1) * gi;
+    const float b = as.b2 * m2[i] + (1.f - as.b2) * gi * gi;
     m1[i] = a;
     m2[i] = b;
-    const float v = p[i] - lr * (a * c1) / (sqrtf(b * c2) + eps);
+    const float v = p[i] - as.lr * (a * as.c1) / (sqrtf(b * as.c2) + as.eps);
     p[i] = v;
     const bf16_t b16 = f2bf(v);
     pbf[i] = b16;
@@ -480,6 +479,75 @@ __global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float 
             pbfT[lt.woff[l] + (long long)k * lt.out[l] + o] = b16;
         }
     }
+}
+
+// Adam from a finished gradient vector g; frozen entries (mask 0: head
+// padding) stay put
+__global__ void __launch_bounds__(256) mlp_adam(float *p, const float *g, float *m1, float *m2, const uint8_t *mask,
+                                                bf16_t *pbf, bf16_t *pbfT, const LayerTable lt, long long n,
+                                                const AdamStep as)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !mask[i])
+        return;
+    adam_update(i, g[i], p, m1, m2, pbf, pbfT, lt, as);
+}
+
+// Adam that also finishes the gradient, in place of the per-layer reduction
+// launches of the backward pass (the training loop's optimizer step).
+// Blocks [0, wblocks): one thread per parameter; a weight's gradient is the
+// sum of its split-K partial products part[z * pstride + i], z = 0 .. splits-1
+// in order (the sum mlp_layer_grads forms).  The following blocks take one
+// bias each, layer by layer: db[c] = sum_m dZt_l[c][m] over the minibatch by
+// the same strided sums and tree as mlp_layer_grads, then thread 0 updates it.
+// The updated parameters equal those of the separate launches bit for bit.
+struct GradSources {
+    const float *part;
+    long long pstride;
+    int splits;
+    const bf16_t *dzt[NLAYER];  // transposed output gradient of each layer, [out][ld]
+    int n, ld;
+};
+
+__global__ void __launch_bounds__(256) mlp_adam_sum(float *p, float *m1, float *m2, const uint8_t *mask, bf16_t *pbf,
+                                                    bf16_t *pbfT, const LayerTable lt, long long n, int wblocks,
+                                                    const GradSources gs, const AdamStep as)
+{
+    if ((int)blockIdx.x < wblocks) {
+        const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+        if (i >= n || !mask[i])
+            return;
+        int l = 0;
+        while (l < NLAYER - 1 && i >= lt.woff[l + 1])
+            ++l;
+        if (i - lt.woff[l] >= (long long)lt.in[l] * lt.out[l])
+            return;  // a bias: the blocks below
+        float s = 0.f;
+        for (int z = 0; z < gs.splits; ++z)
+            s += gs.part[(long long)z * gs.pstride + i];
+        adam_update(i, s, p, m1, m2, pbf, pbfT, lt, as);
+        return;
+    }
+    __shared__ float red[256];
+    int c = (int)blockIdx.x - wblocks, l = 0;
+    while (l < NLAYER - 1 && c >= lt.out[l]) {
+        c -= lt.out[l];
+        ++l;
+    }
+    const bf16_t *dzt = gs.dzt[l];
+    float s = 0.f;
+    for (int m = threadIdx.x; m < gs.n; m += blockDim.x)
+        s += bf2f(dzt[(long long)c * gs.ld + m]);
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int k = 128; k > 0; k >>= 1) {
+        if ((int)threadIdx.x < k)
+            red[threadIdx.x] += red[threadIdx.x + k];
+        __syncthreads();
+    }
+    const long long i = lt.woff[l] + (long long)lt.in[l] * lt.out[l] + c;
+    if (threadIdx.x == 0 && mask[i])
+        adam_update(i, red[0], p, m1, m2, pbf, pbfT, lt, as);
 }
 
 // minibatch gather: row r of the batch is sample perm[r] of the training
@@ -859,6 +927,7 @@ struct clode_mlp {
     Dev<bf16_t> A0, Y[NHID], dY[NHID], dOut;
     Dev<bf16_t> A0t, Yt[NHID], dYt[NHID], dOutT;
     Dev<float> Z[NHID], Zh, X, T, loss, wsp;
+    Dev<float> wall;  // split-K partial weight gradients of all layers (training), KSPLIT_MAX x nparam
     Dev<int> C;
     Dev<unsigned long long> ids;
     long long adamStep = 0;
@@ -968,8 +1037,12 @@ void forward(clode_mlp *h, int n, bool train = false)
 }
 
 // backward from h->dOut / h->dOutT (BF16) into h->g; every product reads
-// operands that are contiguous along its reduction index
-void backward(clode_mlp *h, int n)
+// operands that are contiguous along its reduction index.  With gs (the
+// training loop) the split-K partial weight gradients of every layer stay in
+// h->wall (split z of parameter i at z * nparam + i) and the sums, with the
+// bias gradients, are left to the optimizer launch (mlp_adam_sum); gs
+// receives where they are.  Returns nothing to sum otherwise.
+void backward(clode_mlp *h, int n, GradSources *gs = nullptr)
 {
     const int lt = ldT(n);
     const bf16_t *dz = h->dOut.p, *dzt = h->dOutT.p;
@@ -988,12 +1061,20 @@ void backward(clode_mlp *h, int n)
         w.ksplit = ((n + splits - 1) / splits + TK - 1) / TK * TK;
         const int used = (n + w.ksplit - 1) / w.ksplit;
         const long long count = (long long)Ly.out * Ly.in;
-        w.Cout = h->wsp.p; w.ldc = Ly.in; w.cstride = count;
-        gemm(h, EPI_WGRAD, w);
-        const int sum_blocks = (int)((count + 255) / 256);
-        hipLaunchKernelGGL(mlp_layer_grads, dim3((unsigned int)(sum_blocks + Ly.out)), dim3(256), 0, h->stream,
-                           h->wsp.p, used, count, h->g.p + Ly.woff, sum_blocks, dzt, n, Ly.out, lt, h->g.p + Ly.boff);
-        check(hipGetLastError(), "mlp_layer_grads launch");
+        if (gs) {
+            w.Cout = h->wall.p + Ly.woff; w.ldc = Ly.in; w.cstride = (long long)h->nparam;
+            gemm(h, EPI_WGRAD, w);
+            gs->splits = used;
+            gs->dzt[l] = dzt;
+        } else {
+            w.Cout = h->wsp.p; w.ldc = Ly.in; w.cstride = count;
+            gemm(h, EPI_WGRAD, w);
+            const int sum_blocks = (int)((count + 255) / 256);
+            hipLaunchKernelGGL(mlp_layer_grads, dim3((unsigned int)(sum_blocks + Ly.out)), dim3(256), 0, h->stream,
+                               h->wsp.p, used, count, h->g.p + Ly.woff, sum_blocks, dzt, n, Ly.out, lt,
+                               h->g.p + Ly.boff);
+            check(hipGetLastError(), "mlp_layer_grads launch");
+        }
         if (l == 0)
             break;
         // dY[l-1][m][i] = (sum_o dz[m][o] W[i][o]) * silu'(Z[l-1][m][i])
@@ -1228,6 +1309,7 @@ int clode_mlp_create(clode_mlp **out, int device, unsigned long long seed)
         h->pbf.alloc(h->nparam);
         h->pbfT.alloc(h->nparam);
         h->wsp.alloc((size_t)KSPLIT_MAX * H * H);  // the largest weight gradient, per split
+        h->wall.alloc((size_t)KSPLIT_MAX * h->nparam);  // every layer's, per split (30 MB)
         h->dBff.alloc(2 * NFF);
         h->dtm.alloc(4);
         h->loss.alloc(1);
@@ -1405,6 +1487,14 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
         lt.in[l] = h->L[l].in;
         lt.out[l] = h->L[l].out;
     }
+    // the gradient sums inside the optimizer launch by default (17 launches
+    // per step); CLODE_MLP_FUSED_ADAM=0 sums them in the backward pass (22
+    // launches, same parameters bit for bit)
+    const char *fa = std::getenv("CLODE_MLP_FUSED_ADAM");
+    const bool fused = !(fa && std::strcmp(fa, "0") == 0);
+    int nbias = 0;
+    for (int l = 0; l < NLAYER; ++l)
+        nbias += h->L[l].out;
     long long step = 0;
     check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     for (int e = 0; e < epochs; ++e) {
@@ -1422,16 +1512,31 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
             hipLaunchKernelGGL(mlp_loss, dim3((NHEAD * nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->T.p,
                                h->C.p, h->ids.p, 1.0f / (float)nb, h->dOut.p, h->dOutT.p, ldT(nb), eloss.p);
             check(hipGetLastError(), "mlp_loss launch");
-            backward(h, nb);
+            GradSources gs;
+            std::memset(&gs, 0, sizeof(gs));
+            gs.part = h->wall.p;
+            gs.pstride = (long long)h->nparam;
+            gs.n = nb;
+            gs.ld = ldT(nb);
+            backward(h, nb, fused ? &gs : nullptr);
             // cosine decay to 5 % of the initial rate
             const double frac = (double)step / total;
-            const float lr_t = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac))));
+            AdamStep as;
+            as.lr = (float)(lr * (0.05 + 0.95 * 0.5 * (1.0 + std::cos(3.141592653589793 * frac))));
+            as.b1 = b1;
+            as.b2 = b2;
+            as.eps = eps;
             ++h->adamStep;
-            const float c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)h->adamStep)));
-            const float c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)h->adamStep)));
-            hipLaunchKernelGGL(mlp_adam, dim3((unsigned int)((h->nparam + 255) / 256)), dim3(256), 0, h->stream,
-                               h->p.p, h->g.p, h->m1.p, h->m2.p, h->mask.p, h->pbf.p, h->pbfT.p, lt,
-                               (long long)h->nparam, lr_t, b1, b2, eps, c1, c2);
+            as.c1 = (float)(1.0 / (1.0 - std::pow((double)b1, (double)h->adamStep)));
+            as.c2 = (float)(1.0 / (1.0 - std::pow((double)b2, (double)h->adamStep)));
+            const int wblocks = (int)((h->nparam + 255) / 256);
+            if (fused)
+                hipLaunchKernelGGL(mlp_adam_sum, dim3((unsigned int)(wblocks + nbias)), dim3(256), 0, h->stream,
+                                   h->p.p, h->m1.p, h->m2.p, h->mask.p, h->pbf.p, h->pbfT.p, lt,
+                                   (long long)h->nparam, wblocks, gs, as);
+            else
+                hipLaunchKernelGGL(mlp_adam, dim3((unsigned int)wblocks), dim3(256), 0, h->stream, h->p.p, h->g.p,
+                                   h->m1.p, h->m2.p, h->mask.p, h->pbf.p, h->pbfT.p, lt, (long long)h->nparam, as);
             check(hipGetLastError(), "mlp_adam launch");
         }
         float el = 0;
