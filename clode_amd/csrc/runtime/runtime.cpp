@@ -370,8 +370,9 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
         // every division may multiply by the reciprocal (the hardware one
         // in float, a Newton-refined one in double)
         o.push_back("-freciprocal-math");
-        // math functions may use the hardware approximations (exp as
-        // v_exp_f32 of x log2(e), log from v_log_f32, sqrt from v_sqrt)
+        // math functions may use the hardware approximations (exp and log
+        // from v_exp_f32 and v_log_f32 without the library's extra-precision
+        // steps)
         o.push_back("-fapprox-func");
     } else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
