@@ -27,6 +27,30 @@ def test_exp_double_accuracy():
     assert ulp_err64(got, np.exp(x)).max() <= 2.0
 
 
+def test_exp_double_against_extended_precision_and_thresholds():
+    """exp against x87 extended precision (64-bit significand, about 2^-11 ulp
+    of double): within 1 ulp everywhere and correctly rounded for most
+    arguments.  Around the overflow threshold ln(DBL_MAX) and the underflow
+    threshold -1075 ln 2, which the reduction handles without selects, every
+    double in dense sweeps gives numpy's result bit for bit (inf, DBL_MAX,
+    subnormals, 0), so a dropped or misplaced clamp would show there."""
+    assert np.finfo(np.longdouble).nmant >= 63
+    rng = np.random.default_rng(11)
+    x = np.concatenate([rng.uniform(-708, 709, 300000), rng.uniform(-1, 1, 100000), rng.uniform(-40, 40, 100000)])
+    got = cm_eval("exp", x)
+    ref = np.exp(x.astype(np.longdouble))
+    err = np.abs((got.astype(np.longdouble) - ref) / np.spacing(np.abs(ref.astype(np.float64))).astype(np.longdouble))
+    assert float(err.max()) <= 1.0
+    assert np.mean(err <= 0.5) > 0.9
+    for c in (709.782712893384, -745.1332191019412, -708.3964185322641, -744.44007192138122):
+        sweep = c + np.arange(-5000, 5000) * np.spacing(c)
+        with np.errstate(over="ignore"):
+            want = np.exp(sweep)
+        assert np.array_equal(cm_eval("exp", sweep), want), c
+    # a reduction that lands on r = 0 at the underflow threshold's ulp neighbours
+    assert cm_eval("exp", np.array([-1075 * np.log(2)]))[0] in (0.0, 5e-324)
+
+
 def test_log_double_accuracy():
     rng = np.random.default_rng(2)
     x = np.concatenate([10 ** rng.uniform(-300, 300, 200000), rng.uniform(0.5, 2.0, 100000), 1 + rng.uniform(-1e-8, 1e-8, 1000)])
