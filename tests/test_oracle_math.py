@@ -47,8 +47,18 @@ def test_exp_double_against_extended_precision_and_thresholds():
         with np.errstate(over="ignore"):
             want = np.exp(sweep)
         assert np.array_equal(cm_eval("exp", sweep), want), c
-    # a reduction that lands on r = 0 at the underflow threshold's ulp neighbours
     assert cm_eval("exp", np.array([-1075 * np.log(2)]))[0] in (0.0, 5e-324)
+    # float: the same thresholds ln(FLT_MAX) and -150 ln 2; overflow and
+    # flush to zero exactly where numpy's float32 exp has them, 2 ulp elsewhere
+    for c in (88.72283935546875, -103.97208404541015625, -87.33654):
+        c32 = np.float32(c)
+        sweep = (c32 + np.arange(-3000, 3000, dtype=np.float32) * np.spacing(c32)).astype(np.float64)
+        got = cm_eval("exp", sweep, precision="single")
+        with np.errstate(over="ignore"):
+            want = np.exp(sweep.astype(np.float32)).astype(np.float64)
+        assert np.array_equal(np.isinf(got), np.isinf(want)) and np.array_equal(got == 0, want == 0), c
+        fin = np.isfinite(want) & (want > 0)
+        assert ulp_err32(got[fin], want[fin]).max() <= 2.0, c
 
 
 def test_log_double_accuracy():
