@@ -30,6 +30,7 @@ import argparse
 import inspect
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -447,6 +448,26 @@ def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
     gbs = samples * per_sample / (ms / 1e3) / 1e9
     res["trajectory_2p20x64_fp32c"] = {"kernel_ms": ms, "samples": samples, "bytes_per_sample": per_sample,
                                        "write_GBps": gbs, "frac_of_hbm_peak": gbs / 8000.0}
+    # the practical ceiling of a write-only stream: torch's fill of the same
+    # number of bytes (median of 5 timed fills), in a process of its own (the
+    # torch wheel carries its own HIP runtime, which does not share a process
+    # with the one this library is linked against)
+    fill_script = (
+        "import sys, torch\n"
+        "buf = torch.empty(int(sys.argv[1]) // 4, dtype=torch.float32, device='cuda')\n"
+        "ts = []\n"
+        "for k in range(6):\n"
+        "    e0 = torch.cuda.Event(enable_timing=True); e1 = torch.cuda.Event(enable_timing=True)\n"
+        "    e0.record(); buf.fill_(float(k)); e1.record(); torch.cuda.synchronize()\n"
+        "    ts.append(e0.elapsed_time(e1))\n"
+        "ts = sorted(ts[1:])\n"
+        "print(buf.numel() * 4 / (ts[len(ts) // 2] / 1e3) / 1e9)\n")
+    out = subprocess.run([sys.executable, "-c", fill_script, str(samples * per_sample)], capture_output=True,
+                         text=True, timeout=300, check=True)
+    fill = float(out.stdout.strip().splitlines()[-1])
+    res["fill_same_bytes_GBps"] = fill
+    for k in ("trajectory_65536x1024_fp32c", "trajectory_2p20x64_fp32c"):
+        res[k]["frac_of_fill_rate"] = res[k]["write_GBps"] / fill
     return res
 
 
