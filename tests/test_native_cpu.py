@@ -183,6 +183,13 @@ def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
     prog = s.get_program_string()
     assert "--offload-arch=gfx950" in prog and "-packed-fp32-ops" in prog
     assert f"#define CLODE_PREC {dict(single=1, double=2, fp32c=3)[prec]}" in prog
+    # only fp32c relaxes float semantics; single and double programs must
+    # round exactly as the oracle does
+    relaxed = ("-fgpu-flush-denormals-to-zero", "-freciprocal-math", "-fapprox-func",
+               "-fno-hip-fp32-correctly-rounded-divide-sqrt")
+    for flag in relaxed:
+        assert (flag in prog) == (prec == "fp32c"), flag
+    assert ("-fhip-fp32-correctly-rounded-divide-sqrt" in prog) == (prec != "fp32c")
 
 
 _COMPILE_INTO = """
