@@ -462,9 +462,13 @@ def lorenz_c3(n=1 << 20, t_end=500.0, n_traj=65536, store=1024):
         "    ts.append(e0.elapsed_time(e1))\n"
         "ts = sorted(ts[1:])\n"
         "print(buf.numel() * 4 / (ts[len(ts) // 2] / 1e3) / 1e9)\n")
-    out = subprocess.run([sys.executable, "-c", fill_script, str(samples * per_sample)], capture_output=True,
-                         text=True, timeout=300, check=True)
-    fill = float(out.stdout.strip().splitlines()[-1])
+    try:
+        out = subprocess.run([sys.executable, "-c", fill_script, str(samples * per_sample)], capture_output=True,
+                             text=True, timeout=300, check=True)
+        fill = float(out.stdout.strip().splitlines()[-1])
+    except (subprocess.SubprocessError, OSError, ValueError, IndexError) as e:
+        res["fill_same_bytes_error"] = f"{type(e).__name__}: {e}"[:300]
+        return res
     res["fill_same_bytes_GBps"] = fill
     for k in ("trajectory_65536x1024_fp32c", "trajectory_2p20x64_fp32c"):
         res[k]["frac_of_fill_rate"] = res[k]["write_GBps"] / fill
