@@ -778,24 +778,29 @@ def main():
             extra[name] = {"error": f"{type(e).__name__}: {e}"}
         print(f"# {name}: {time.perf_counter() - t0:.1f} s", file=sys.stderr, flush=True)
 
+    r32 = None
     if "fp32c" in wanted:
-        r32 = run_lactotroph("fp32c", args.steps, args.warmup)
-        print(f"# fp32c: {r32['member_steps_per_s']:.4e} member-steps/s, {r32['ms_per_step']:.1f} ms/step",
-              file=sys.stderr, flush=True)
-        a32, _ = kernel_tflops(r32, "features")
-        extra["fp32c"] = {
-            "member_steps_per_s": r32["member_steps_per_s"], "ms_per_step": r32["ms_per_step"],
-            "kernel_ms": r32["kernel_ms"], "speedup_vs_fp64_wall": r64["ms_per_step"] / r32["ms_per_step"],
-            "features_kernel_tflops": a32, "features_kernel_frac_of_fp32_peak": a32 / PEAK_TFLOPS["f32"],
-            "feature_error_vs_fp64": feature_error(
-                lactotroph_step_features("fp32c", r64["x0_step"]),
-                lactotroph_step_features("double", r64["x0_step"]), r64["feature_names"],
-                lactotroph_step_features("double", r64["x0_step"], 1e-8, 1e-10)),
-            "feature_error_members": "all 65,536, one step from the same initial states (fresh simulators)",
-        }
+        def fp32c_extra():
+            nonlocal r32
+            r32 = run_lactotroph("fp32c", args.steps, args.warmup)
+            print(f"# fp32c: {r32['member_steps_per_s']:.4e} member-steps/s, {r32['ms_per_step']:.1f} ms/step",
+                  file=sys.stderr, flush=True)
+            a32, _ = kernel_tflops(r32, "features")
+            return {
+                "member_steps_per_s": r32["member_steps_per_s"], "ms_per_step": r32["ms_per_step"],
+                "kernel_ms": r32["kernel_ms"], "speedup_vs_fp64_wall": r64["ms_per_step"] / r32["ms_per_step"],
+                "features_kernel_tflops": a32, "features_kernel_frac_of_fp32_peak": a32 / PEAK_TFLOPS["f32"],
+                "feature_error_vs_fp64": feature_error(
+                    lactotroph_step_features("fp32c", r64["x0_step"]),
+                    lactotroph_step_features("double", r64["x0_step"]), r64["feature_names"],
+                    lactotroph_step_features("double", r64["x0_step"], 1e-8, 1e-10)),
+                "feature_error_members": "all 65,536, one step from the same initial states (fresh simulators)",
+            }
+
+        run_extra("fp32c", fp32c_extra)
     if "seq" in wanted:
         run_extra("one_launch_sequence_fp64", lambda: one_launch_sequence("double", args.steps, r64))
-        if "fp32c" in wanted:
+        if r32 is not None:
             run_extra("one_launch_sequence_fp32c", lambda: one_launch_sequence("fp32c", args.steps, r32))
     if "c1" in wanted:
         run_extra("vdp_4096_rk4_fp64", vdp_c1)
