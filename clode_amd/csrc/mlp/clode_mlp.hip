@@ -175,18 +175,21 @@ __device__ inline void gemm_stage(bf16_t *S, const u16x8 (&v)[TM * (KC / 8) / 25
     }
 }
 
+// workgroup-tile product of one block (bx, by, bz) of the grid
+// (ceil(N / TN), ceil(M / TM), splits); smem holds (TM + TN) x (KC + KPAD)
+// BF16 elements
 template <int EPI, int T>
-__global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
+__device__ __attribute__((always_inline)) inline void gemm_block(const GemmArgs &g, int bx, int by, int bz,
+                                                                 bf16_t *smem)
 {
     constexpr int TM = Tile<T>::TM, TN = Tile<T>::TN, KC = Tile<T>::KC, WB = Tile<T>::WB;
     constexpr int KCP = KC + KPAD, LDT = TM * (KC / 8) / 256;
-    __shared__ __attribute__((aligned(16))) bf16_t smem[(TM + TN) * KCP];
     bf16_t *As = smem, *Bs = smem + TM * KCP;
     const int t = threadIdx.x;
     const int lane = t & 63, wave = t >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
-    const int kbeg = g.ksplit > 0 ? (int)blockIdx.z * g.ksplit : 0;
+    const int m0 = by * TM, n0 = bx * TN;
+    const int kbeg = g.ksplit > 0 ? bz * g.ksplit : 0;
     const int kend = g.ksplit > 0 ? min(g.K, kbeg + g.ksplit) : g.K;
     f32x4 acc[WB][WB];
 #pragma unroll
@@ -266,7 +269,7 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
                     if (stage_t)
                         Tt[nl * TP + ml] = y;
                 } else {
-                    g.Cout[(long long)blockIdx.z * g.cstride + (long long)m * g.ldc + n] = v;
+                    g.Cout[(long long)bz * g.cstride + (long long)m * g.ldc + n] = v;
                 }
             }
     if (stage_t) {  // uniform across the workgroup
@@ -287,6 +290,29 @@ __global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
             }
         }
     }
+}
+
+template <int EPI, int T>
+__global__ void __launch_bounds__(256) mlp_gemm(const GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t smem[(Tile<T>::TM + Tile<T>::TN) * (Tile<T>::KC + KPAD)];
+    gemm_block<EPI, T>(g, (int)blockIdx.x, (int)blockIdx.y, (int)blockIdx.z, smem);
+}
+
+// two independent products in one launch (a layer's weight gradient and its
+// input gradient): blocks [0, n1) are the first product's grid (gx1 x gy1 x
+// splits, x fastest), the rest the second's (gx2 wide); each block computes
+// exactly what it computes in a launch of its own
+template <int EPI1, int EPI2, int T>
+__global__ void __launch_bounds__(256) mlp_gemm_pair(const GemmArgs g1, int gx1, int gy1, int n1, const GemmArgs g2,
+                                                     int gx2)
+{
+    __shared__ __attribute__((aligned(16))) bf16_t smem[(Tile<T>::TM + Tile<T>::TN) * (Tile<T>::KC + KPAD)];
+    const int b = (int)blockIdx.x;
+    if (b < n1)
+        gemm_block<EPI1, T>(g1, b % gx1, (b / gx1) % gy1, b / (gx1 * gy1), smem);
+    else
+        gemm_block<EPI2, T>(g2, (b - n1) % gx2, (b - n1) / gx2, 0, smem);
 }
 
 // ---------------------------------------------------------------- encode
@@ -985,16 +1011,41 @@ void gemm_tiled(clode_mlp *h, int epi, const GemmArgs &a)
 // is 128 tiles of 64 x 64 on 256 CUs).  Every tile size accumulates each
 // output element over the same K chunks in the same order, so the results
 // are identical.
-void gemm(clode_mlp *h, int epi, const GemmArgs &a)
+int tile_for(const clode_mlp *h, const GemmArgs &a)
 {
     const long long splits = a.ksplit > 0 ? (a.K + a.ksplit - 1) / a.ksplit : 1;
     const long long tiles64 = (long long)((a.M + 63) / 64) * ((a.N + 63) / 64) * splits;
     if (a.ksplit == 0 && a.M >= 32768)
-        gemm_tiled<128>(h, epi, a);
-    else if (tiles64 < h->cus)
-        gemm_tiled<32>(h, epi, a);
-    else
-        gemm_tiled<64>(h, epi, a);
+        return 128;
+    return tiles64 < h->cus ? 32 : 64;
+}
+
+void gemm(clode_mlp *h, int epi, const GemmArgs &a)
+{
+    switch (tile_for(h, a)) {
+    case 128: gemm_tiled<128>(h, epi, a); break;
+    case 32: gemm_tiled<32>(h, epi, a); break;
+    default: gemm_tiled<64>(h, epi, a); break;
+    }
+}
+
+// a layer's weight gradient w (split-K) and input gradient d (x SiLU') in
+// one launch when both use 32 x 32 tiles (the training minibatches), as two
+// launches otherwise; the same products either way
+void gemm_pair(clode_mlp *h, const GemmArgs &w, const GemmArgs &d)
+{
+    if (tile_for(h, w) != 32 || tile_for(h, d) != 32) {
+        gemm(h, EPI_WGRAD, w);
+        gemm(h, EPI_BWD_DSILU, d);
+        return;
+    }
+    const int gx1 = (w.N + 31) / 32, gy1 = (w.M + 31) / 32;
+    const int sp1 = w.ksplit > 0 ? (w.K + w.ksplit - 1) / w.ksplit : 1;
+    const int gx2 = (d.N + 31) / 32, gy2 = (d.M + 31) / 32;
+    const int n1 = gx1 * gy1 * sp1;
+    hipLaunchKernelGGL((mlp_gemm_pair<EPI_WGRAD, EPI_BWD_DSILU, 32>), dim3((unsigned int)(n1 + gx2 * gy2)),
+                       dim3(256), 0, h->stream, w, gx1, gy1, n1, d, gx2);
+    check(hipGetLastError(), "mlp_gemm_pair launch");
 }
 
 GemmArgs ga() { GemmArgs a; std::memset(&a, 0, sizeof(a)); return a; }
@@ -1063,9 +1114,26 @@ void backward(clode_mlp *h, int n, GradSources *gs = nullptr)
         const long long count = (long long)Ly.out * Ly.in;
         if (gs) {
             w.Cout = h->wall.p + Ly.woff; w.ldc = Ly.in; w.cstride = (long long)h->nparam;
-            gemm(h, EPI_WGRAD, w);
             gs->splits = used;
             gs->dzt[l] = dzt;
+            if (l == 0) {
+                gemm(h, EPI_WGRAD, w);
+                break;
+            }
+            // the weight gradient and the input gradient read the same
+            // output gradient and nothing the other writes: one launch
+            GemmArgs d = ga();
+            d.M = n; d.N = Ly.in; d.K = Ly.out;
+            d.A = dz; d.lda_m = nout; d.lda_k = 1;
+            d.B = h->pbfT.p + Ly.woff; d.ldb_k = 1; d.ldb_n = Ly.out;
+            d.Zin = h->Z[l - 1].p; d.ldz = H;
+            d.Yout = h->dY[l - 1].p; d.ldy = H;
+            d.YTout = h->dYt[l - 1].p; d.ldt = lt;
+            gemm_pair(h, w, d);
+            dz = h->dY[l - 1].p;
+            dzt = h->dYt[l - 1].p;
+            nout = H;
+            continue;
         } else {
             w.Cout = h->wsp.p; w.ldc = Ly.in; w.cstride = count;
             gemm(h, EPI_WGRAD, w);
@@ -1487,10 +1555,11 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
         lt.in[l] = h->L[l].in;
         lt.out[l] = h->L[l].out;
     }
-    // the gradient sums inside the optimizer launch by default (17 launches
-    // per step); CLODE_MLP_FUSED_ADAM=0 sums them in the backward pass (22
-    // launches, same parameters bit for bit)
-    const char *fa = std::getenv("CLODE_MLP_FUSED_ADAM");
+    // by default each layer's weight and input gradients share a launch and
+    // the gradient sums run inside the optimizer launch (13 launches per
+    // step); CLODE_MLP_FUSED_TRAIN=0 launches every product on its own and
+    // sums in the backward pass (22 launches, same parameters bit for bit)
+    const char *fa = std::getenv("CLODE_MLP_FUSED_TRAIN");
     const bool fused = !(fa && std::strcmp(fa, "0") == 0);
     int nbias = 0;
     for (int l = 0; l < NLAYER; ++l)

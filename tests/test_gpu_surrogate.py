@@ -130,24 +130,26 @@ def test_training_fits_a_smooth_map(gpu_available):
 
 @pytest.mark.parametrize("n, batch", [(16384, 1024), (5000, 2048), (3000, 64)])
 def test_fused_optimizer_matches_separate_sums(gpu_available, n, batch):
-    """The training loop's optimizer launch that also sums the split-K weight
-    gradients and reduces the bias gradients (mlp_adam_sum) against the
-    backward pass's own reduction launches followed by a plain Adam launch
-    (CLODE_MLP_FUSED_ADAM=0): same partial products, summation order and
-    tree, so the trained parameters agree bit for bit (the reported epoch
-    losses are summed with atomics, in no fixed order, so only to rounding).
-    Full and ragged last minibatches, and minibatches too small to split."""
+    """The training loop's fused launches (each layer's weight and input
+    gradients in one launch, mlp_gemm_pair; the split-K sums and bias
+    reductions inside the optimizer launch, mlp_adam_sum) against every
+    product in a launch of its own, the backward pass's reduction launches
+    and a plain Adam launch (CLODE_MLP_FUSED_TRAIN=0): same blocks, partial
+    products, summation order and tree, so the trained parameters agree bit
+    for bit (the reported epoch losses are summed with atomics, in no fixed
+    order, so only to rounding).  Full and ragged last minibatches, and
+    minibatches too small to split."""
     rng = np.random.default_rng(8)
     U = rng.random((n, 2))
     T, cls = smooth_targets(U)
     fused = net(13)
     h_fused = fused.train(U, T, cls, epochs=3, batch=batch, lr=2e-3, seed=6)
-    os.environ["CLODE_MLP_FUSED_ADAM"] = "0"
+    os.environ["CLODE_MLP_FUSED_TRAIN"] = "0"
     try:
         separate = net(13)
         h_sep = separate.train(U, T, cls, epochs=3, batch=batch, lr=2e-3, seed=6)
     finally:
-        del os.environ["CLODE_MLP_FUSED_ADAM"]
+        del os.environ["CLODE_MLP_FUSED_TRAIN"]
     assert np.array_equal(fused.get_params(), separate.get_params())
     assert np.allclose(h_fused, h_sep, rtol=1e-5, atol=0)
     assert not np.array_equal(fused.get_params(), net(13).get_params())  # training moved the parameters
