@@ -89,3 +89,45 @@ def test_random_configuration_bitexact(gpu_available, seed):
     p.assert_bitexact(("F", "xf", "dt", "tf"))
     if model in NOISY:
         assert np.array_equal(np.array(p.gpu.get_rng_state(), dtype=np.uint64), p.orc.get_rng())
+
+
+def draw_trajectory(seed):
+    rng = np.random.default_rng(1000 + seed)
+    model = ["van_der_pol", "lorenz", "lactotroph", "lactotroph_noise", "ornstein_uhlenbeck"][seed % 5]
+    if model in NOISY:
+        stepper = "seuler"
+    else:
+        stepper = (ADAPTIVE + FIXED)[rng.integers(6)]
+        if stepper in FIXED and model == "lactotroph":
+            stepper = "rk4"  # with the small fixed step below
+    prec = ["double", "single"][rng.integers(2)]
+    n = int(rng.integers(17, 140))
+    _, _, data, length = MODELS[model]
+    start = [0.0, 1000.0, 0.25, 1.0 - 2.0 ** -20][rng.integers(4)]
+    rtol = float(10 ** (rng.uniform(-8, -4) if prec == "double" else rng.uniform(-5, -3)))
+    dt = {"seuler": 0.05, "rk4": 0.01 if model != "lactotroph" else 0.002}.get(stepper, 0.01)
+    if stepper in ADAPTIVE:
+        dt = float(10 ** rng.uniform(-3, -1))
+    nout = int(rng.integers(1, 6))
+    max_store = int(rng.integers(1, 300))
+    sp = (dt, 1.0, rtol * 1e-2, rtol, 200000, max_store, nout)
+    return model, stepper, prec, n, data(rng, n), (start, start + length * rng.uniform(0.1, 0.5)), sp, rng
+
+
+@pytest.mark.parametrize("seed", list(range(16)))
+def test_random_trajectory_bitexact(gpu_available, seed):
+    """Randomly drawn trajectory launches (model, stepper, precision, output
+    stride nout, max_store, time origin, noise with a drawn seed): every
+    stored sample (t, x, dx, aux), the stored counts and the final state agree
+    with the oracle bit for bit."""
+    model, stepper, prec, n, (x0, pars), tspan, sp, rng = draw_trajectory(seed)
+    print(f"trajectory seed {seed}: {model} {stepper} {prec} n={n} tspan={tspan} sp={sp}", flush=True)
+    p = Pair(model, stepper, "", prec, kind="trajectory", sp=sp)
+    p.set_problem_data(x0, pars)
+    if model in NOISY:
+        p.seed(int(rng.integers(0, 2 ** 31 - 1)))
+    p.set_tspan(*tspan)
+    p.trajectory()
+    p.assert_trajectory_bitexact()
+    if model in NOISY:
+        assert np.array_equal(np.array(p.gpu.get_rng_state(), dtype=np.uint64), p.orc.get_rng())
