@@ -17,7 +17,9 @@ HIP-event kernel times and the algorithmic FLOPs of clode_amd/perfmodel.py,
 and a CPU baseline: the CPU oracle (the reference-semantics restatement,
 not the reference binary, which cannot run here) timed on a member sample.
 
-Extra workloads (--extras, default all): fp32c (the headline in fp32c), c1
+Extra workloads (--extras, default all): fp32c (the headline in fp32c), seq
+(the step as one launch), replay (the step with the features pass replaying
+the warm-up's recorded steps from HBM), c1
 (Van der Pol 4096, rk4), noise (the lactotroph Euler-Maruyama variant),
 c3 (Lorenz 2^20, and the 65,536 x 1024-sample trajectory), c4 (mixed 2^20
 batched sweep), c5 (surrogate-screened 2048 x 2048 map).
@@ -163,6 +165,60 @@ def one_launch_sequence(precision, steps, r_three=None):
         res["three_launch_kernel_ms"] = three
         res["kernel_speedup"] = three / res["kernel_ms"]
         res["wall_speedup"] = r_three["ms_per_step"] / res["ms_per_step"]
+    return res
+
+
+def replay_step(precision, steps, r_plain=None):
+    """The same step with the features replay on: the thresh2 warm-up pass
+    records every member's accepted steps in HBM (time, state, derivative,
+    aux, step size as doubles: 88 B per member-step in FP64, 120 B in fp32c)
+    and the features pass reads them back instead of integrating them again.
+    Wall time per step with the feature readback, the kernel times, the
+    record's HBM bytes and read rate, the member-steps actually integrated,
+    and whether the features equal those of the integrating step bit for
+    bit (both from the same initial states)."""
+    rec = 8 * (2 + 2 * 4 + 1 + (4 if precision == "fp32c" else 0))
+    budget = 65536 * rec * 24000  # every member's warm-up (at most ~21,600 steps here)
+    ref = lactotroph_sim(precision)
+    one_step(ref)
+    want = ref.get_observer_results().to_ndarray()
+    del ref
+    sim = lactotroph_sim(precision)
+    sim.set_replay_budget(budget)
+    sim.enable_step_counter(True)
+    one_step(sim)
+    same = bool(np.array_equal(sim.get_observer_results().to_ndarray(), want))
+    cap, replayed = sim.replay_info()
+    sim.reset_step_counts()
+    kernel_ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        kernel_ms.append(one_step(sim))
+    wall = time.perf_counter() - t0
+    integrated = int(sim.get_step_counts().sum())
+    # one more step, instrumented: the warm-up's recorded steps
+    sim.reset_step_counts()
+    sim.transient(t_span=(0.0, 5000.0))
+    tr = int(sim.get_step_counts().sum())
+    sim.set_tspan((0.0, 20000.0))
+    sim.initialize_observer()
+    warm = int(sim.get_step_counts().sum()) - tr
+    k_rec = sim.last_kernel_ms()
+    sim.features(initialize_observer=False, fetch_results=False)
+    k_rep = sim.last_kernel_ms()
+    sim.set_replay_budget(0)
+    k = np.median(np.array(kernel_ms), axis=0)
+    res = {"ms_per_step": 1e3 * wall / steps,
+           "kernel_ms": {"transient": float(k[0]), "warmup_record": float(k[1]), "features_replay": float(k[2])},
+           "integrated_member_steps_per_s": integrated / wall,
+           "records_per_member": cap, "replayed": replayed,
+           "record_bytes": warm * rec, "record_bytes_per_member_step": rec,
+           "replay_read_GBps": warm * rec / (k_rep / 1e3) / 1e9,
+           "record_write_GBps_during_warmup": warm * rec / (k_rec / 1e3) / 1e9,
+           "identical_features_to_integrating_step": same}
+    if r_plain is not None:
+        res["wall_speedup_vs_integrating_step"] = r_plain["ms_per_step"] / res["ms_per_step"]
+        res["features_kernel_speedup"] = r_plain["kernel_ms"]["features"] / res["kernel_ms"]["features_replay"]
     return res
 
 
@@ -688,8 +744,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extra", action="store_true")
     ap.add_argument("--cpu-members", type=int, default=1024)
-    ap.add_argument("--extras", default="fp32c,seq,c1,noise,c3,c4,c5",
-                    help="comma list of extra workloads: fp32c, seq (one-launch sequence), c1, noise, c3, c4, c5 "
+    ap.add_argument("--extras", default="fp32c,seq,replay,c1,noise,c3,c4,c5",
+                    help="comma list of extra workloads: fp32c, seq (one-launch sequence), replay (features "
+                         "replay), c1, noise, c3, c4, c5 "
                          "(or 'none'); --no-extra "
                          "drops them all")
     ap.add_argument("--c4-rounds", type=int, default=8)
@@ -802,6 +859,10 @@ def main():
         run_extra("one_launch_sequence_fp64", lambda: one_launch_sequence("double", args.steps, r64))
         if r32 is not None:
             run_extra("one_launch_sequence_fp32c", lambda: one_launch_sequence("fp32c", args.steps, r32))
+    if "replay" in wanted:
+        run_extra("features_replay_fp64", lambda: replay_step("double", args.steps, r64))
+        if r32 is not None:
+            run_extra("features_replay_fp32c", lambda: replay_step("fp32c", args.steps, r32))
     if "c1" in wanted:
         run_extra("vdp_4096_rk4_fp64", vdp_c1)
     if "noise" in wanted:

@@ -58,6 +58,13 @@ struct ClodeArgs {
     unsigned int q_stride;       // stride through the order (1 = every member)
     unsigned int q_slice;        // loop iterations between queue checks
     unsigned int q_pad;
+    // warm-up record (replay kernels only): the warm-up pass writes up to
+    // replay_cap records per member and their count, the features pass reads
+    // them back instead of integrating those steps again
+    double *replay;              // [replay_cap / CHUNK][nPts][CHUNK][record]
+    unsigned int *replay_n;      // [nPts] records written
+    unsigned int replay_cap;     // records per member (a multiple of the chunk)
+    unsigned int replay_pad;
 };
 
 // Arguments of the one-launch transient -> features sequence (clode_sequence):

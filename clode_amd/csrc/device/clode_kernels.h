@@ -249,7 +249,62 @@ struct TransientLane {
 #define CLODE_TWO_PASS 0
 #endif
 
-struct InitObsLane {
+// Warm-up record of a member's accepted steps (replay programs only; the
+// layout is clode_replay_record's): record k holds the member after its
+// (k+1)-th accepted step, everything the features pass's observer and, after
+// the last record, its integration read from the member.
+#define CLODE_REC_LEN (2 + 2 * N_VAR + N_AUX + (CLODE_PREC == 3 ? N_VAR : 0))
+static_assert(CLODE_REC_LEN == clode_replay_record(N_VAR, N_AUX, CLODE_PREC), "warm-up record layout");
+
+CLODE_DEV double *replay_record(const ClodeArgs &a, unsigned int i, unsigned int k)
+{
+    const size_t chunk = (size_t)(k / CLODE_REPLAY_CHUNK) * a.nPts + i;
+    return a.replay + (chunk * CLODE_REPLAY_CHUNK + k % CLODE_REPLAY_CHUNK) * CLODE_REC_LEN;
+}
+
+CLODE_DEV void replay_write(const ClodeArgs &a, unsigned int i, unsigned int k, const Member &m)
+{
+    double *r = replay_record(a, i, k);
+    r[0] = (double)m.ti;
+#pragma unroll
+    for (int j = 0; j < N_VAR; ++j) {
+        r[1 + j] = (double)m.xi[j];
+        r[1 + N_VAR + j] = (double)m.dxi[j];
+    }
+#pragma unroll
+    for (int j = 0; j < N_AUX; ++j)
+        r[1 + 2 * N_VAR + j] = (double)m.auxi[j];
+    r[1 + 2 * N_VAR + N_AUX] = (double)m.dt;
+#if CLODE_PREC == 3
+#pragma unroll
+    for (int j = 0; j < N_VAR; ++j)
+        r[2 + 2 * N_VAR + N_AUX + j] = (double)m.xc[j];
+#endif
+}
+
+CLODE_DEV void replay_read(const ClodeArgs &a, unsigned int i, unsigned int k, Member &m)
+{
+    const double *r = replay_record(a, i, k);
+    m.ti = (timetype)r[0];
+#pragma unroll
+    for (int j = 0; j < N_VAR; ++j) {
+        m.xi[j] = (realtype)r[1 + j];
+        m.dxi[j] = (realtype)r[1 + N_VAR + j];
+    }
+#pragma unroll
+    for (int j = 0; j < N_AUX; ++j)
+        m.auxi[j] = (realtype)r[1 + 2 * N_VAR + j];
+    m.dt = (decltype(m.dt))r[1 + 2 * N_VAR + N_AUX];
+#if CLODE_PREC == 3
+#pragma unroll
+    for (int j = 0; j < N_VAR; ++j)
+        m.xc[j] = (realtype)r[2 + 2 * N_VAR + N_AUX + j];
+#endif
+}
+
+// REC: also write the warm-up record (replay programs)
+template <bool REC>
+struct InitObsLaneT {
     Member m;
     Observer ob;
     ObserverParamsR op;
@@ -275,11 +330,15 @@ struct InitObsLane {
         ++trials;
         if (!CLODE_STEP(m))
             return;  // trial rejected: retried on the next iteration
+        if (REC && step < a.replay_cap)
+            replay_write(a, i, step, m);
         ++step;
         ob.warmup(m.ti, m.xi, m.dxi, m.auxi, op);
     }
     CLODE_DEV void finish(const ClodeArgs &a)
     {
+        if (REC)
+            a.replay_n[i] = step < a.replay_cap ? step : a.replay_cap;
         if (CLODE_TWO_PASS) {
             m.ti = t0;
             load_state(m, a, i);
@@ -291,13 +350,24 @@ struct InitObsLane {
         ob.fields(st);
     }
 };
+typedef InitObsLaneT<false> InitObsLane;
 
-struct FeaturesLane {
+// REPLAY: the first replay_n[i] accepted steps are read back from the
+// warm-up record instead of integrated.  The warm-up pass integrates the
+// same member from the same state, step size and span with the same
+// controller, and its loop (t < t1) ends no later than this one (t <= t1),
+// so record k is exactly the member after this pass's (k+1)-th step; the
+// observer sees the same values in the same order, and after the last record
+// the member (state, derivative, aux, time and step size; the controller
+// holds nothing else between steps) continues integrating where the warm-up
+// record ends.  Only replayed steps are left out of the step counts.
+template <bool REPLAY>
+struct FeaturesLaneT {
     Member m;
     Observer ob;
     ObserverParamsR op;
     timetype t0, t1;
-    unsigned int i, step, trials;
+    unsigned int i, step, trials, nrep;
     bool terminal;
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
@@ -314,14 +384,19 @@ struct FeaturesLane {
         getRHS((realtype)m.ti, m.xi, m.p, m.dxi, m.auxi, m.wi);
         step = 0;
         trials = 0;
+        nrep = REPLAY ? a.replay_n[i] : 0u;
         terminal = false;
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const { return !terminal && m.ti <= t1 && step < sp.max_steps; }
     CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
     {
-        ++trials;
-        if (!CLODE_STEP(m))
-            return;  // trial rejected: retried on the next iteration
+        if (REPLAY && step < nrep) {
+            replay_read(a, i, step, m);
+        } else {
+            ++trials;
+            if (!CLODE_STEP(m))
+                return;  // trial rejected: retried on the next iteration
+        }
         const ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
         ++step;
         ob.update(m.ti, m.xi, m.dxi, m.auxi, op, io);
@@ -337,9 +412,10 @@ struct FeaturesLane {
         ObsStore st{io};
         ob.fields(st);
         store_member(m, a, i);
-        count_steps(a, i, step, trials);
+        count_steps(a, i, REPLAY ? step - (step < nrep ? step : nrep) : step, trials);
     }
 };
+typedef FeaturesLaneT<false> FeaturesLane;
 #endif  // CLODE_OBSERVER
 
 CLODE_DEV void trajectory_store(const ClodeArgs &a, unsigned int i, unsigned int s, const Member &m)
@@ -501,6 +577,22 @@ extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS
 {
     run_queue<FeaturesLane>(a);
 }
+
+#if defined(CLODE_REPLAY) && CLODE_TWO_PASS
+// Warm-up recording and features replay (grid launches): features(true) as
+// these two launches gives the results of clode_initialize_observer and
+// clode_features bit for bit, with the second pass's integration replaced by
+// reading the first pass's record back from HBM.
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_initialize_observer_record(const ClodeArgs a)
+{
+    run_grid<InitObsLaneT<true>>(a);
+}
+
+extern "C" __global__ void __launch_bounds__(CLODE_BLOCK_MAX) CLODE_KERNEL_ATTRS clode_features_replay(const ClodeArgs a)
+{
+    run_grid<FeaturesLaneT<true>>(a);
+}
+#endif
 
 // transient -> x0 <- xf -> observer warm-up and initialisation -> features,
 // member by member in one launch: the same lanes as the four separate calls,

@@ -75,4 +75,18 @@ static inline constexpr bool clode_obs_two_pass(int obs)
     return obs == CLODE_OBS_THRESH2 || obs == CLODE_OBS_NHOOD2;
 }
 
+// Warm-up record for the features replay of a two-pass observer: for every
+// accepted step of the warm-up pass, the member's time, state, derivative,
+// aux values and next step size, and in fp32c (prec 3) the state's
+// compensation term, each as a double (every precision's values widen to
+// double exactly).  A member's records are grouped in chunks of
+// CLODE_REPLAY_CHUNK consecutive steps, and chunk c of member i is at
+// (c * nPts + i) * CLODE_REPLAY_CHUNK * record doubles: the lanes of a wave,
+// which replay their steps in lockstep, read neighbouring chunks.
+#define CLODE_REPLAY_CHUNK 8
+static inline constexpr int clode_replay_record(int nVar, int nAux, int prec)
+{
+    return 2 + 2 * nVar + nAux + (prec == 3 ? nVar : 0);
+}
+
 #endif  // CLODE_LAYOUT_H

@@ -389,6 +389,14 @@ public:
         check(clode_get_precision(sim_, &p));
         return p;
     }
+    void set_replay_budget(unsigned long long bytes) { check(clode_set_replay_budget(sim_, bytes)); }
+    py::tuple get_replay_info()
+    {
+        unsigned int cap = 0;
+        int replayed = 0;
+        check(clode_get_replay_info(sim_, &cap, &replayed));
+        return py::make_tuple(cap, replayed != 0);
+    }
     void set_step_counter(bool on) { check(clode_set_step_counter(sim_, on ? 1 : 0)); }
     void reset_step_counts() { check(clode_reset_step_counts(sim_)); }
     std::vector<unsigned long long> get_step_counts()
@@ -852,6 +860,8 @@ PYBIND11_MODULE(clode_cpp_wrapper, m)
         .def("set_dt_refresh", &SimulatorBasePy::set_dt_refresh)
         .def("last_kernel_ms", &SimulatorBasePy::last_kernel_ms)
         .def("last_launch_variant", &SimulatorBasePy::last_launch_variant)
+        .def("set_replay_budget", &SimulatorBasePy::set_replay_budget)
+        .def("get_replay_info", &SimulatorBasePy::get_replay_info)
         .def("set_step_counter", &SimulatorBasePy::set_step_counter)
         .def("reset_step_counts", &SimulatorBasePy::reset_step_counts)
         .def("get_step_counts", &SimulatorBasePy::get_step_counts)

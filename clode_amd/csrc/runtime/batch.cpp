@@ -96,6 +96,7 @@ void Batch::prepare()
             throw InvalidArgument("all simulators of a batch must use the same precision");
         s->requireData("batch");
         s->ensureStream();
+        ++s->inputGen;  // batch launches write the member buffers: no replay of an earlier record
         if (features_)
             s->resizeFeaturesVariables();
         s->makeArgs();  // allocates the step counters if enabled, outside any capture

@@ -734,6 +734,18 @@ int clode_last_launch_variant(clode_sim *sim, int *waves_per_simd)
     CHECK_SIM(sim) CLODE_TRY *need(waves_per_simd, "waves_per_simd") = sim->s->lastVariantWaves;
     CLODE_CATCH
 }
+int clode_set_replay_budget(clode_sim *sim, unsigned long long bytes)
+{
+    CHECK_SIM(sim) CLODE_TRY sim->s->setReplayBudget((size_t)bytes);
+    CLODE_CATCH
+}
+int clode_get_replay_info(clode_sim *sim, unsigned int *records_per_member, int *replayed)
+{
+    CHECK_SIM(sim) CLODE_TRY
+    *need(records_per_member, "records_per_member") = sim->s->replayCap;
+    *need(replayed, "replayed") = sim->s->lastReplay ? 1 : 0;
+    CLODE_CATCH
+}
 int clode_set_step_counter(clode_sim *sim, int enable)
 {
     CHECK_SIM(sim) CLODE_TRY sim->s->setStepCounter(enable != 0);

@@ -89,6 +89,7 @@ struct ProgramConfig {
     int eVar = 0, fVar = 0;  // observer event / feature variable
     int nVar = 0, nPar = 0, nAux = 0, nWiener = 0, nStore = 0;
     int wavesPerEU = 0;  // > 0: kernels compiled for at least this many waves per SIMD
+    bool replay = false;  // also the warm-up record / features replay kernels
     std::string rhs;
     std::string rhs_name;
 };

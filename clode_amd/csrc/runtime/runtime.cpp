@@ -242,6 +242,8 @@ std::string compose_source(const ProgramConfig &cfg)
     s << "#define N_STORE_EVENTS " << cfg.nStore << "\n";
     if (cfg.wavesPerEU > 0)
         s << "#define CLODE_KERNEL_ATTRS __attribute__((amdgpu_waves_per_eu(" << cfg.wavesPerEU << ")))\n";
+    if (cfg.replay)
+        s << "#define CLODE_REPLAY 1\n";
     s << "#include \"clode_prelude.h\"\n";
     s << "#pragma clang force_cuda_host_device begin\n";
     // basename only: the program text (and its cache key) must not depend on

@@ -60,6 +60,7 @@ Simulator::~Simulator()
 // CLODE::setProblemInfo (CLODE.cpp:43-59)
 void Simulator::setProblemInfo(const ProblemInfo &pi)
 {
+    ++inputGen;
     if (pi.nVar <= 0)
         throw InvalidArgument("the problem must have at least one state variable");
     if (pi.nPar < 0 || pi.nAux < 0 || pi.nWiener < 0)
@@ -73,6 +74,7 @@ void Simulator::setProblemInfo(const ProblemInfo &pi)
 // CLODE::setStepper (CLODE.cpp:61-76): unknown names leave the stepper unchanged
 void Simulator::setStepper(const std::string &s)
 {
+    ++inputGen;
     if (stepper_map().find(s) != stepper_map().end())
         stepper = s;
     else
@@ -83,6 +85,7 @@ void Simulator::setStepper(const std::string &s)
 // released; the caller sets problem data again, as with the reference.
 void Simulator::setPrecision(int p)
 {
+    ++inputGen;
     if (p < PREC_SINGLE || p > PREC_FP32C)
         throw InvalidArgument("precision must be 1 (single), 2 (double) or 3 (fp32c)");
     if (p != prec) {
@@ -98,6 +101,7 @@ void Simulator::setPrecision(int p)
 // CLODE::setOpenCL (CLODE.cpp:88-102)
 void Simulator::setDevice(int d)
 {
+    ++inputGen;
     if (d != device) {
         if (stream) {
             hipSetDevice(device);
@@ -152,12 +156,13 @@ std::shared_ptr<Program> Simulator::currentProgram()
     return program;
 }
 
-std::shared_ptr<Program> Simulator::occupancyProgram(int waves)
+std::shared_ptr<Program> Simulator::variantProgram(int waves, bool replay)
 {
     ProgramConfig c = programConfig();
     c.wavesPerEU = waves;
+    c.replay = replay;
     std::string key = compose_source(c);
-    auto &slot = programsW[waves];
+    auto &slot = programsW[2 * waves + (replay ? 1 : 0)];
     if (!slot.second || key != slot.first) {
         slot.second = get_program(c, root.empty() ? std::string() : root);
         slot.first = key;
@@ -177,9 +182,9 @@ std::shared_ptr<Program> Simulator::occupancyProgram(int waves)
 // than that (FP64 Lorenz dopri5 at 4 waves) spills enough to lose 26 %.
 // Every variant performs the same arithmetic in the same order.
 // CLODE_OCCUPANCY_VARIANT=0 in the environment keeps the default kernel.
-hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads, unsigned int block)
+hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads, unsigned int block, bool replay)
 {
-    hipFunction_t fn = currentProgram()->function(device, kernel);
+    hipFunction_t fn = (replay ? variantProgram(0, true) : currentProgram())->function(device, kernel);
     lastVariantWaves = 0;
     static const bool enabled = [] {
         const char *e = std::getenv("CLODE_OCCUPANCY_VARIANT");
@@ -196,7 +201,7 @@ hipFunction_t Simulator::gridFunction(const char *kernel, unsigned int nThreads,
     if (waves < 1 || waves >= 4 || (size_t)nThreads < simds * 64 * (size_t)(waves + 1))
         return fn;
     lastVariantWaves = waves + 1;
-    return occupancyProgram(waves + 1)->function(device, kernel);
+    return variantProgram(waves + 1, replay)->function(device, kernel);
 }
 
 // CLODE::buildCL (CLODE.cpp:155-171) + the subclasses' kernel creation
@@ -291,6 +296,7 @@ void Simulator::downloadInto(const DeviceBuffer &b, size_t n, size_t elem, doubl
 // CLODE::setNpts (CLODE.cpp:174-242)
 void Simulator::setNpts(int newNpts)
 {
+    ++inputGen;
     if (newNpts <= 0)
         throw InvalidArgument("the ensemble must have at least one member");
     if (newNpts == nPts)
@@ -324,6 +330,7 @@ void Simulator::setNpts(int newNpts)
 // CLODE::setProblemData (CLODE.cpp:245-280)
 void Simulator::setProblemData(const std::vector<double> &nx0, const std::vector<double> &npars)
 {
+    ++inputGen;
     if (nx0.size() % prob.nVar != 0) {
         log_msg(LOG_INFO, "Invalid initial condition vector: not a multiple of nVar=" + std::to_string(prob.nVar));
         log_msg(LOG_INFO, "...Initial conditions were not updated!");
@@ -349,6 +356,7 @@ void Simulator::setProblemData(const std::vector<double> &nx0, const std::vector
 // CLODE::setX0 (CLODE.cpp:283-315)
 void Simulator::setX0(const std::vector<double> &nx0)
 {
+    ++inputGen;
     if (nPts > 0 && nx0.size() == (size_t)nPts * prob.nVar) {
         x0 = nx0;
         upload(d_x0, x0, wideSize());
@@ -360,6 +368,7 @@ void Simulator::setX0(const std::vector<double> &nx0)
 // CLODE::setPars (CLODE.cpp:318-350)
 void Simulator::setPars(const std::vector<double> &np)
 {
+    ++inputGen;
     if (nPts > 0 && np.size() == (size_t)nPts * prob.nPar) {
         pars = np;
         upload(d_pars, pars, realSize());
@@ -373,6 +382,7 @@ void Simulator::setPars(const std::vector<double> &np)
 // CLODE::setTspan (CLODE.cpp:352-375)
 void Simulator::setTspan(const std::vector<double> &t)
 {
+    ++inputGen;
     if (t.size() != 2)
         throw InvalidArgument("tspan must have two elements");
     tspan = t;
@@ -382,6 +392,7 @@ void Simulator::setTspan(const std::vector<double> &t)
 // but the device buffer is not (unless dt refresh is enabled)
 void Simulator::setSolverParams(const SolverParams &nsp)
 {
+    ++inputGen;
     if (nsp.nout == 0)
         throw InvalidArgument("nout must be positive");
     sp = nsp;
@@ -393,6 +404,7 @@ void Simulator::setSolverParams(const SolverParams &nsp)
 // CLODE::seedRNG() (CLODE.cpp:420-444)
 void Simulator::seedRNG()
 {
+    ++inputGen;
     if (nPts <= 0)
         return;
     std::random_device rd;
@@ -409,6 +421,7 @@ void Simulator::seedRNG()
 // CLODE::seedRNG(int) (CLODE.cpp:447-464): word k = seed + k in int arithmetic
 void Simulator::seedRNG(int seed)
 {
+    ++inputGen;
     if (nPts <= 0)
         return;
     for (size_t k = 0; k < rngHost.size(); ++k) {
@@ -423,6 +436,7 @@ void Simulator::seedRNG(int seed)
 
 void Simulator::setMemberTspan(const double *t, size_t n)
 {
+    ++inputGen;
     if (!t) {
         memberTspan = false;
         d_tspan_m.release();
@@ -528,14 +542,14 @@ unsigned int queue_grid(hipFunction_t fn, int device, unsigned int block, size_t
     return (unsigned int)std::max<size_t>(1, std::min(resident, needed));
 }
 
-void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads)
+void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads, bool replay)
 {
     auto p = currentProgram();
     ensureStream();
     unsigned int block = blockSize > 0 ? (unsigned int)blockSize : 64u;
     ClodeArgs args = a;
     void *params[] = {&args};
-    if (launchMode == 1) {
+    if (launchMode == 1 && !replay) {
         // work queue: "clode_X" -> "clode_queue_X", counter cleared by a kernel
         std::string qname = std::string("clode_queue_") + (kernel + 6);
         hipFunction_t fn = p->function(device, qname.c_str());
@@ -559,7 +573,7 @@ void Simulator::launch(const char *kernel, const ClodeArgs &a, unsigned int nThr
         lastKernelMs = ms;
         return;
     }
-    hipFunction_t fn = gridFunction(kernel, nThreads, block);
+    hipFunction_t fn = gridFunction(kernel, nThreads, block, replay);
     unsigned int grid = (nThreads + block - 1) / block;
     hip_check(hipEventRecord(ev0, stream), "hipEventRecord");
     hip_check(hipModuleLaunchKernel(fn, grid, 1, 1, block, 1, 1, 0, stream, params, nullptr),
@@ -614,6 +628,7 @@ std::vector<unsigned long long> Simulator::getRngState()
 // CLODE::transient (CLODE.cpp:468-493)
 void Simulator::transient()
 {
+    ++inputGen;
     requireData("transient");
     launch("clode_transient", makeArgs(), (unsigned int)nPts);
     log_msg(LOG_INFO, "run transient");
@@ -629,6 +644,7 @@ void Simulator::shiftTspan()
 // CLODE::shiftX0 (CLODE.cpp:502-515): device-to-device copy xf -> x0
 void Simulator::shiftX0()
 {
+    ++inputGen;
     requireData("shift_x0");
     ensureStream();
     hip_check(hipMemcpyAsync(d_x0.ptr, d_xf.ptr, (size_t)prob.nVar * nPts * wideSize(), hipMemcpyDeviceToDevice,
@@ -678,6 +694,7 @@ void Simulator::updateObserverInfo()
 // CLODEfeatures::setObserverParams (CLODEfeatures.cpp:78-106)
 void Simulator::setObserverParams(const ObserverParams &nop)
 {
+    ++inputGen;
     if (nop.eVarIx >= (unsigned int)prob.nVar || nop.fVarIx >= (unsigned int)prob.nVar)
         throw InvalidArgument("observer eVarIx/fVarIx must index a state variable");
     op = nop;
@@ -687,6 +704,7 @@ void Simulator::setObserverParams(const ObserverParams &nop)
 // CLODEfeatures::setObserver (CLODEfeatures.cpp:63-76)
 void Simulator::setObserver(const std::string &name)
 {
+    ++inputGen;
     if (observer_map().find(name) != observer_map().end()) {
         observer = name;
         updateObserverInfo();
@@ -714,12 +732,74 @@ void Simulator::resizeFeaturesVariables()
     }
 }
 
+void Simulator::setReplayBudget(size_t bytes)
+{
+    replayBudget = bytes;
+    replayGen = ~0ull;
+    replayCap = 0;
+    if (bytes == 0) {
+        d_replay.release();
+        d_replay_n.release();
+    }
+}
+
+bool Simulator::replayUsable() const
+{
+    return kind == 1 && replayBudget > 0 && nPts > 0 && clode_obs_two_pass(observerId()) && prob.nWiener == 0 &&
+           launchMode == 0;
+}
+
+// Records per member: what the budget holds, at most max_steps (rounded up
+// to whole chunks), a multiple of the chunk; the buffer grows when needed and
+// is kept.  false (no recording) below one chunk or when the allocation fails.
+bool Simulator::prepareReplay()
+{
+    const size_t rec = (size_t)clode_replay_record(prob.nVar, prob.nAux, prec) * 8;
+    const size_t chunk = CLODE_REPLAY_CHUNK;
+    size_t cap = replayBudget / ((size_t)nPts * rec);
+    const size_t steps = ((size_t)sp.max_steps + chunk - 1) / chunk * chunk;
+    cap = std::min(std::min(cap, steps), (size_t)0xfffffff0u) / chunk * chunk;
+    replayCap = 0;
+    if (cap < chunk)
+        return false;
+    const size_t bytes = cap * (size_t)nPts * rec;
+    try {
+        if (d_replay.bytes < bytes)
+            d_replay.allocate(device, bytes);
+        if (d_replay_n.bytes < (size_t)nPts * 4)
+            d_replay_n.allocate(device, (size_t)nPts * 4);
+    } catch (const std::exception &e) {
+        log_msg(LOG_WARN, std::string("features replay off: ") + e.what());
+        d_replay.release();
+        d_replay_n.release();
+        (void)hipGetLastError();
+        return false;
+    }
+    replayCap = (unsigned int)cap;
+    return true;
+}
+
+ClodeArgs Simulator::replayArgs()
+{
+    ClodeArgs a = makeArgs();
+    a.replay = d_replay.as<double>();
+    a.replay_n = d_replay_n.as<unsigned int>();
+    a.replay_cap = replayCap;
+    return a;
+}
+
 // CLODEfeatures::initializeObserver (CLODEfeatures.cpp:180-212)
 void Simulator::initializeObserver()
 {
     requireData("initialize_observer");
     resizeFeaturesVariables();
-    launch("clode_initialize_observer", makeArgs(), (unsigned int)nPts);
+    replayGen = ~0ull;
+    if (replayUsable() && prepareReplay()) {
+        launch("clode_initialize_observer_record", replayArgs(), (unsigned int)nPts, true);
+        replayGen = inputGen;
+    } else {
+        launch("clode_initialize_observer", makeArgs(), (unsigned int)nPts);
+    }
     observerInitialized = true;
 }
 
@@ -737,7 +817,12 @@ void Simulator::features()
     resizeFeaturesVariables();
     if (!observerInitialized)
         initializeObserver();
-    launch("clode_features", makeArgs(), (unsigned int)nPts);
+    lastReplay = replayGen == inputGen && replayUsable() && d_replay.ptr;
+    if (lastReplay)
+        launch("clode_features_replay", replayArgs(), (unsigned int)nPts, true);
+    else
+        launch("clode_features", makeArgs(), (unsigned int)nPts);
+    ++inputGen;  // xf, dt and tf are stored: the record describes no later run
 }
 
 // transient(), shift_x0(), features(true) in one launch: each lane runs its
@@ -745,6 +830,7 @@ void Simulator::features()
 // global spans only (per-member spans would give both passes the same span).
 void Simulator::transientFeatures(double tr0, double tr1, double fe0, double fe1)
 {
+    ++inputGen;
     requireData("transient_features");
     if (kind != 1)
         throw InvalidArgument("transient_features needs a feature simulator");
@@ -843,6 +929,7 @@ void Simulator::resizeTrajectoryVariables()
 // CLODEtrajectory::trajectory (CLODEtrajectory.cpp:97-130)
 void Simulator::trajectory()
 {
+    ++inputGen;
     requireData("trajectory");
     resizeTrajectoryVariables();
     launch("clode_trajectory", makeArgs(), (unsigned int)nPts);

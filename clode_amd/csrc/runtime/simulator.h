@@ -107,6 +107,16 @@ public:
     int lastVariantWaves = 0;
     bool stepCounter = false;
 
+    // Features replay, off unless a budget is set: with a two-pass observer
+    // (thresh2, nhood2) and no noise, the warm-up pass also records every
+    // member's accepted steps, up to `bytes` of device memory, and the
+    // features pass that follows it with nothing changed in between reads
+    // them back instead of integrating them again (bit-identical results).
+    void setReplayBudget(size_t bytes);
+    size_t replayBudget = 0;
+    unsigned int replayCap = 0;  // records per member of the last recording (0: none)
+    bool lastReplay = false;     // the last features pass replayed a record
+
     void setStepCounter(bool on);
     void resetStepCounts();
     // row 0: accepted steps per member; row 1: step trials (a step and its rejected trials)
@@ -119,6 +129,13 @@ private:
     DeviceBuffer d_odr, d_odu, d_F;
     DeviceBuffer d_t, d_x, d_dx, d_aux, d_nStored;
     DeviceBuffer d_work, d_tspan_m, d_steps, d_qorder;
+    DeviceBuffer d_replay, d_replay_n;
+    // inputGen counts the calls that change what an integration reads; a
+    // record is replayed only while it still equals replayGen
+    unsigned long long inputGen = 0, replayGen = ~0ull;
+    bool replayUsable() const;
+    bool prepareReplay();
+    ClodeArgs replayArgs();
     size_t odrFields = 0, oduFields = 0, Felements = 0;
     bool memberTspan = false;
     hipStream_t stream = nullptr;
@@ -131,11 +148,12 @@ private:
     int observerId() const;
     ProgramConfig programConfig() const;
     std::shared_ptr<Program> currentProgram();
-    // the same program compiled for at least `waves` waves per SIMD, used by
-    // grid launches of large ensembles (see gridFunction)
-    std::shared_ptr<Program> occupancyProgram(int waves);
+    // the same program compiled for at least `waves` waves per SIMD (0: the
+    // default compilation), used by grid launches of large ensembles (see
+    // gridFunction), and with the replay kernels when `replay` is set
+    std::shared_ptr<Program> variantProgram(int waves, bool replay);
     std::map<int, std::pair<std::string, std::shared_ptr<Program>>> programsW;
-    hipFunction_t gridFunction(const char *kernel, unsigned int nThreads, unsigned int block);
+    hipFunction_t gridFunction(const char *kernel, unsigned int nThreads, unsigned int block, bool replay = false);
     void ensureStream();
     void setNpts(int n);
     void upload(DeviceBuffer &b, const std::vector<double> &v, size_t elem);
@@ -144,7 +162,7 @@ private:
     void downloadInto(const DeviceBuffer &b, size_t n, size_t elem, double *out);
     DeviceBuffer d_Fm;  // member-major copy of F for getFMembers
     ClodeArgs makeArgs();
-    void launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads);
+    void launch(const char *kernel, const ClodeArgs &a, unsigned int nThreads, bool replay = false);
     void updateObserverInfo();
     void resizeFeaturesVariables();
     void resizeTrajectoryVariables();

@@ -216,6 +216,22 @@ class FeatureSimulator(Simulator):
         """Run the observer's initialisation (warm-up pass for two-pass observers)."""
         self._integrator.initialize_observer()
 
+    def set_replay_budget(self, nbytes: int) -> None:
+        """Spend up to ``nbytes`` of device memory on the features replay:
+        for a two-pass observer (threshold_2, neighbourhood_2) on a system
+        without noise, the observer's warm-up pass records every member's
+        accepted steps and the features pass that follows it reads them back
+        instead of integrating them a second time (bit-identical results;
+        8 (2 + 2 nVar + nAux) bytes per member-step, 8 nVar more in fp32c).
+        0 turns it off and frees the buffer."""
+        self._integrator.set_replay_budget(int(nbytes))
+
+    def replay_info(self) -> Tuple[int, bool]:
+        """(records per member of the last warm-up recording, whether the last
+        features pass replayed it)."""
+        cap, replayed = self._integrator.get_replay_info()
+        return int(cap), bool(replayed)
+
     def features(self, t_span: Optional[Tuple[float, float]] = None, initialize_observer: Optional[bool] = None,
                  update_x0: bool = True, fetch_results: bool = True) -> Optional[ObserverOutput]:
         """Integrate with feature detection; optionally return the features."""

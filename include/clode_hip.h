@@ -222,6 +222,22 @@ int clode_last_kernel_ms(clode_sim *sim, double *ms);
  * launches of ensembles with more waves of members than the default kernel
  * keeps resident per SIMD use (no reference counterpart: a diagnostic). */
 int clode_last_launch_variant(clode_sim *sim, int *waves_per_simd);
+/* Features replay: HBM capacity spent instead of a second integration.  With
+ * a budget of `bytes` > 0, a feature simulator whose observer has a warm-up
+ * pass (thresh2, nhood2) and whose system has no noise records each member's
+ * accepted warm-up steps (time, state, derivative, aux values and next step
+ * size, as doubles: 8 (2 + 2 nVar + nAux) bytes per member-step, 8 nVar more
+ * in fp32c) up to the budget, and the features pass that follows the warm-up
+ * with nothing changed in between reads those steps back instead of
+ * integrating them a second time; a member whose steps did not all fit
+ * integrates on from its last record.  Results are bit-identical.  0 (the
+ * default) turns it off and frees the buffer.  Grid launches only (the work
+ * queue mode integrates both passes).  No reference counterpart: the
+ * reference integrates both passes (CLODEfeatures.cpp:180-259). */
+int clode_set_replay_budget(clode_sim *sim, unsigned long long bytes);
+/* Records per member of the last warm-up recording (0: none) and whether the
+ * last features pass replayed it (1) or integrated (0). */
+int clode_get_replay_info(clode_sim *sim, unsigned int *records_per_member, int *replayed);
 /* Compile-only check of the current configuration (no device needed). */
 int clode_compile(clode_sim *sim);
 /* Per-member count of accepted steps (member-steps), accumulated over every
