@@ -282,9 +282,17 @@ CLODE_DEV void replay_write(const ClodeArgs &a, unsigned int i, unsigned int k, 
 #endif
 }
 
-CLODE_DEV void replay_read(const ClodeArgs &a, unsigned int i, unsigned int k, Member &m)
+// record k into registers (issued ahead of its use by the replay's ring)
+CLODE_DEV void replay_load(const ClodeArgs &a, unsigned int i, unsigned int k, double r[CLODE_REC_LEN])
 {
-    const double *r = replay_record(a, i, k);
+    const double *p = replay_record(a, i, k);
+#pragma unroll
+    for (int f = 0; f < CLODE_REC_LEN; ++f)
+        r[f] = p[f];
+}
+
+CLODE_DEV void replay_take(const double r[CLODE_REC_LEN], Member &m)
+{
     m.ti = (timetype)r[0];
 #pragma unroll
     for (int j = 0; j < N_VAR; ++j) {
@@ -361,6 +369,16 @@ typedef InitObsLaneT<false> InitObsLane;
 // the member (state, derivative, aux, time and step size; the controller
 // holds nothing else between steps) continues integrating where the warm-up
 // record ends.  Only replayed steps are left out of the step counts.
+// The records are loaded CLODE_REPLAY_DEPTH steps ahead of their use into a
+// ring of registers (slot k mod depth holds record k), so that the loads of
+// the next steps are in flight while the observer works on this one.
+// Measured on the lactotroph (65,536 members, one wave per SIMD): depths 1,
+// 2 and 4 replay in the same time (26.3-26.6 ms); the compiler's waits on
+// the ring's loop-carried registers let no more than one step overlap, so
+// the default is the one-step lookahead with the fewest registers.
+#ifndef CLODE_REPLAY_DEPTH
+#define CLODE_REPLAY_DEPTH 1
+#endif
 template <bool REPLAY>
 struct FeaturesLaneT {
     Member m;
@@ -369,6 +387,7 @@ struct FeaturesLaneT {
     timetype t0, t1;
     unsigned int i, step, trials, nrep;
     bool terminal;
+    double ring[REPLAY ? CLODE_REPLAY_DEPTH : 1][CLODE_REC_LEN];
     CLODE_DEV void begin(const ClodeArgs &a, unsigned int idx)
     {
         i = idx;
@@ -386,22 +405,51 @@ struct FeaturesLaneT {
         trials = 0;
         nrep = REPLAY ? a.replay_n[i] : 0u;
         terminal = false;
+        if (REPLAY) {
+#pragma unroll
+            for (int d = 0; d < CLODE_REPLAY_DEPTH; ++d)
+                if ((unsigned int)d < nrep)
+                    replay_load(a, i, (unsigned int)d, ring[d]);
+        }
     }
     CLODE_DEV bool running(const SolverParamsR &sp) const { return !terminal && m.ti <= t1 && step < sp.max_steps; }
-    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    // one accepted step: the observer's update and event test
+    CLODE_DEV void observe(const ClodeArgs &a)
     {
-        if (REPLAY && step < nrep) {
-            replay_read(a, i, step, m);
-        } else {
-            ++trials;
-            if (!CLODE_STEP(m))
-                return;  // trial rejected: retried on the next iteration
-        }
         const ObsIO io{(ostore *)a.odr, a.odu, a.nPts, i, 0, 0};
         ++step;
         ob.update(m.ti, m.xi, m.dxi, m.auxi, op, io);
         if (ob.eventFunction(m.ti, m.xi, m.dxi, m.auxi, op))
             terminal = ob.computeEventFeatures(m.ti, m.xi, m.dxi, m.auxi, op, io);
+    }
+    CLODE_DEV void advance(const ClodeArgs &a, const SolverParamsR &sp)
+    {
+        if (REPLAY && step < nrep) {
+            // the member's whole replay, in bursts of depth steps: slot d
+            // holds record start + d (bursts start at multiples of the
+            // depth), the unrolled slots keep the ring in registers, and
+            // every slot's next load is issued (a valid record, the last one
+            // near the end) so that each burst leaves the same loads in
+            // flight
+            bool go = true;
+            while (go) {
+#pragma unroll
+                for (int d = 0; d < (REPLAY ? CLODE_REPLAY_DEPTH : 1); ++d) {
+                    if (go) {
+                        replay_take(ring[d], m);
+                        const unsigned int ahead = step + CLODE_REPLAY_DEPTH;
+                        replay_load(a, i, ahead < nrep ? ahead : nrep - 1u, ring[d]);
+                        observe(a);
+                        go = step < nrep && running(sp);
+                    }
+                }
+            }
+            return;
+        }
+        ++trials;
+        if (!CLODE_STEP(m))
+            return;  // trial rejected: retried on the next iteration
+        observe(a);
     }
     CLODE_DEV void finish(const ClodeArgs &a)
     {

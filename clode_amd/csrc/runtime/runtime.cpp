@@ -378,6 +378,14 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
         o.push_back("-fapprox-func");
     } else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
+    // the replay kernels unroll the features pass's observer once per ring
+    // slot; at the default threshold the observer's own per-variable loops
+    // then stay rolled, and their variable indices send the whole lane to
+    // scratch memory
+    if (cfg.replay) {
+        o.push_back("-mllvm");
+        o.push_back("-unroll-threshold=1000");
+    }
     // code-generation tuning: CLODE_HIPRTC_OPTIONS holds extra whitespace-
     // separated compiler options (part of the program cache key)
     if (const char *extra = std::getenv("CLODE_HIPRTC_OPTIONS")) {
