@@ -218,7 +218,13 @@ void Simulator::build()
     log_msg(LOG_INFO, "built program for device " + std::to_string(device));
 }
 
-void Simulator::compileOnly() { currentProgram(); }
+void Simulator::compileOnly()
+{
+    currentProgram();
+    // with a replay budget, also the program with the record / replay kernels
+    if (kind == 1 && replayBudget > 0 && clode_obs_two_pass(observerId()) && prob.nWiener == 0)
+        variantProgram(0, true);
+}
 
 std::string Simulator::programString()
 {

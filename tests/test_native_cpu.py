@@ -192,6 +192,28 @@ def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
     assert ("-fhip-fp32-correctly-rounded-divide-sqrt" in prog) == (prec != "fp32c")
 
 
+@pytest.mark.parametrize("stepper,observer,prec", [("dopri5", "thresh2", "double"), ("bs23", "nhood2", "single"),
+                                                    ("rk4", "thresh2", "fp32c")])
+def test_replay_program_compiles_for_gfx950(stepper, observer, prec):
+    """With a replay budget, compile() also builds the program holding the
+    warm-up record and features replay kernels (on the CPU, for gfx950); the
+    replay state reads (0 records, not replayed) until a warm-up runs, and
+    the C ABI refuses null outputs."""
+    from clode_amd.cpp import clode_cpp_wrapper as w
+
+    pi = w.ProblemInfo(os.path.join(MODELS, "lactotroph.cl"), ["v", "n", "f", "c"], ["gcal", "gsk", "gbk", "gk"],
+                       ["ica"], 0)
+    s = w.FeatureSimulatorBase(pi, stepper, observer, w.ObserverParams(), prec, w.OpenCLResource(), "")
+    assert s.get_replay_info() == (0, False)
+    s.set_replay_budget(1 << 30)
+    s.compile()
+    s.set_replay_budget(0)
+    assert s.get_replay_info() == (0, False)
+    lib = ctypes.CDLL(LIB)
+    assert lib.clode_set_replay_budget(None, ctypes.c_ulonglong(1)) == 1
+    assert lib.clode_get_replay_info(None, None, None) == 1
+
+
 _COMPILE_INTO = """
 import sys
 from clode_amd.cpp import clode_cpp_wrapper as w
