@@ -770,9 +770,9 @@ bool Simulator::prepareReplay()
         return false;
     const size_t bytes = cap * (size_t)nPts * rec;
     try {
-        if (d_replay.bytes < bytes)
+        if (d_replay.bytes < bytes || d_replay.device != device)
             d_replay.allocate(device, bytes);
-        if (d_replay_n.bytes < (size_t)nPts * 4)
+        if (d_replay_n.bytes < (size_t)nPts * 4 || d_replay_n.device != device)
             d_replay_n.allocate(device, (size_t)nPts * 4);
     } catch (const std::exception &e) {
         log_msg(LOG_WARN, std::string("features replay off: ") + e.what());
