@@ -159,3 +159,33 @@ def test_lactotroph_2p17_occupancy_variant_sample_bitexact(gpu_available):
     o.set_tspan((0.0, 300.0))
     o.features()
     assert np.array_equal(F[idx], o.get("F").reshape(-1, idx.size).T)
+
+
+@pytest.mark.parametrize("precision", ["double", "fp32c"])
+def test_lactotroph_256x256_replay_equals_integration(gpu_available, precision):
+    """The full 256 x 256 grid with the features replay on (the warm-up's
+    record of every member's steps, about 2,700 per member here) gives the
+    integrating path's features, final states, step sizes and times bit for
+    bit, in FP64 and in fp32c."""
+    res = {}
+    for budget in (0, 24 << 30):
+        sim = clode.FeatureSimulator(src_file=os.path.join(MODELS, "lactotroph.cl"),
+                                     variables={"v": -60.0, "n": 0.0, "f": 0.0, "c": 0.1},
+                                     parameters={"gcal": 1.5, "gsk": 3.0, "gbk": 1.0, "gk": 2.0}, aux=["ica"],
+                                     observer=clode.Observer.threshold_2, stepper=clode.Stepper.dormand_prince,
+                                     single_precision=precision, dt=0.1, dtmax=1.0, abstol=1e-8, reltol=1e-6,
+                                     max_steps=10 ** 8, t_span=(0.0, 1000.0), observer_x_up_thresh=0.3,
+                                     observer_x_down_thresh=0.2)
+        gk, gbk = np.linspace(0, 10, 256), np.linspace(0, 2, 256)
+        GK, GBK = np.meshgrid(gk, gbk, indexing="ij")
+        sim.set_ensemble(parameters={"gk": GK, "gbk": GBK})
+        sim.set_replay_budget(budget)
+        sim.transient()
+        sim.set_tspan((0.0, 3000.0))
+        F = sim.features(update_x0=False).to_ndarray()
+        res[budget] = (F, sim.get_final_state(), sim.get_dt(), sim.replay_info())
+        sim.set_replay_budget(0)
+    (f0, x0, dt0, info0), (f1, x1, dt1, info1) = res[0], res[24 << 30]
+    assert info0 == (0, False) and info1[1] and info1[0] >= 3000
+    assert np.array_equal(f0, f1, equal_nan=True)
+    assert np.array_equal(x0, x1) and np.array_equal(dt0, dt1)
