@@ -214,6 +214,7 @@ def replay_step(precision, steps, r_plain=None):
            "records_per_member": cap, "replayed": replayed,
            "record_bytes": warm * rec, "record_bytes_per_member_step": rec,
            "replay_read_GBps": warm * rec / (k_rep / 1e3) / 1e9,
+           "replay_read_frac_of_hbm_peak": warm * rec / (k_rep / 1e3) / 1e9 / 8000.0,
            "record_write_GBps_during_warmup": warm * rec / (k_rec / 1e3) / 1e9,
            "identical_features_to_integrating_step": same}
     if r_plain is not None:
