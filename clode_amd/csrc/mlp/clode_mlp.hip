@@ -1543,9 +1543,12 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
     upload(h, Call.p, cls, n * 4);
     h->ensureRows((size_t)batch);
     std::mt19937_64 rng(seed);
-    std::vector<unsigned int> perm(n);
+    // epoch e's order is a shuffle of epoch e-1's; the next epoch's shuffle
+    // runs on the host while the GPU trains on the current one
+    std::vector<unsigned int> perm(n), next(n);
     for (size_t i = 0; i < n; ++i)
         perm[i] = (unsigned int)i;
+    std::shuffle(perm.begin(), perm.end(), rng);
     const size_t steps_per_epoch = (n + batch - 1) / batch;
     const double total = (double)steps_per_epoch * epochs;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -1567,7 +1570,6 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
     long long step = 0;
     check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     for (int e = 0; e < epochs; ++e) {
-        std::shuffle(perm.begin(), perm.end(), rng);
         upload(h, dperm.p, perm.data(), n * 4);
         check(hipMemsetAsync(eloss.p, 0, 4, h->stream), "hipMemset");
         for (size_t s = 0; s < steps_per_epoch; ++s, ++step) {
@@ -1608,10 +1610,15 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
                                    h->m1.p, h->m2.p, h->mask.p, h->pbf.p, h->pbfT.p, lt, (long long)h->nparam, as);
             check(hipGetLastError(), "mlp_adam launch");
         }
+        if (e + 1 < epochs) {
+            next = perm;  // the buffer being uploaded stays untouched
+            std::shuffle(next.begin(), next.end(), rng);
+        }
         float el = 0;
         download(h, &el, eloss.p, 4);
         if (epoch_loss)
             epoch_loss[e] = el / (float)steps_per_epoch;
+        perm.swap(next);
     }
     check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");

@@ -376,6 +376,14 @@ std::vector<std::string> compile_options(const ProgramConfig &cfg)
         // from v_exp_f32 and v_log_f32 without the library's extra-precision
         // steps)
         o.push_back("-fapprox-func");
+        // the hardware's non-IEEE mode (which clang allows only with NaNs
+        // assumed absent): min and max then need no canonicalising of their
+        // operands, which IEEE mode spends an instruction on for every
+        // loop-carried value.  Results of finite members are unchanged (the
+        // features and final states of the lactotroph and Lorenz benchmark
+        // ensembles compare bit for bit); NaN members may differ.
+        o.push_back("-fno-honor-nans");
+        o.push_back("-mno-amdgpu-ieee");
     } else
         o.push_back("-fhip-fp32-correctly-rounded-divide-sqrt");
     // the replay kernels unroll the features pass's observer once per ring

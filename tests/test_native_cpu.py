@@ -186,7 +186,7 @@ def test_hiprtc_compiles_for_gfx950(kind, stepper, observer, prec):
     # only fp32c relaxes float semantics; single and double programs must
     # round exactly as the oracle does
     relaxed = ("-fgpu-flush-denormals-to-zero", "-freciprocal-math", "-fapprox-func",
-               "-fno-hip-fp32-correctly-rounded-divide-sqrt")
+               "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-fno-honor-nans", "-mno-amdgpu-ieee")
     for flag in relaxed:
         assert (flag in prog) == (prec == "fp32c"), flag
     assert ("-fhip-fp32-correctly-rounded-divide-sqrt" in prog) == (prec != "fp32c")
