@@ -409,13 +409,19 @@ __global__ void __launch_bounds__(256) mlp_loss(const float *Zh, int n, const fl
             float mx = zh[2];
             for (int c = 1; c < NCLS; ++c)
                 mx = fmaxf(mx, zh[2 + c]);
+            // each class's exponential once, for the sum and the gradient
+            float ex[NCLS];
             float se = 0.f;
-            for (int c = 0; c < NCLS; ++c)
-                se += __expf(zh[2 + c] - mx);
+#pragma unroll
+            for (int c = 0; c < NCLS; ++c) {
+                ex[c] = __expf(zh[2 + c] - mx);
+                se += ex[c];
+            }
             const int y = cls[i];
             l += (mx + __logf(se)) - zh[2 + y];
+#pragma unroll
             for (int c = 0; c < NCLS; ++c) {
-                const float p = __expf(zh[2 + c] - mx) / se;
+                const float p = ex[c] / se;
                 d[2 + c] = f2bf((p - (c == y ? 1.f : 0.f)) * inv_n);
             }
         }
