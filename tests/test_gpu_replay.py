@@ -7,8 +7,9 @@ reference's CLODEfeatures::initializeObserver and features do,
 clode/cpp/CLODEfeatures.cpp:180-259): the record's capacity limit (members
 that integrate on from their last record), a maximum event count that ends
 members in the middle of their record, per-member spans, the explicit
-initialize_observer() + features(False) sequence, and a record invalidated
-by a change between the two calls.
+initialize_observer() + features(False) sequence, a record invalidated
+by a change between the two calls, and the randomly drawn parity
+configurations that have a two-pass observer.
 """
 import numpy as np
 import pytest
@@ -189,3 +190,38 @@ def test_replay_not_used_with_noise(gpu_available):
     assert p.gpu.get_replay_info() == (0, False)
     p.assert_bitexact(("F", "xf", "dt", "tf"))
     assert np.array_equal(np.array(p.gpu.get_rng_state(), dtype=np.uint64), p.orc.get_rng())
+
+
+def _replay_seeds():
+    from test_gpu_parity_random import NOISY as RNOISY, draw
+
+    seeds = []
+    for seed in range(48):
+        model, _, observer, *_ = draw(seed)
+        if observer in ("thresh2", "nhood2") and model not in RNOISY:
+            seeds.append(seed)
+    return seeds
+
+
+@pytest.mark.parametrize("seed", _replay_seeds())
+def test_replay_random_configuration_bitexact(gpu_available, seed):
+    """The randomly drawn parity configurations with a two-pass observer and
+    no noise (every stepper, spans before and across zero, per-member spans,
+    event lists), run with the replay on: bit for bit the oracle's."""
+    from test_gpu_parity_random import draw
+
+    model, stepper, observer, prec, n, (x0, pars), tspan, sp, per_member, rng = draw(seed)
+    print(f"replay seed {seed}: {model} {stepper} {observer} {prec} n={n} tspan={tspan} per_member={per_member}",
+          flush=True)
+    p = Pair(model, stepper, observer, prec, obs=dict(max_event_timestamps=int(rng.integers(0, 4))), sp=sp)
+    p.gpu.set_replay_budget(1 << 30)
+    p.set_problem_data(x0, pars)
+    p.set_tspan(*tspan)
+    if per_member:
+        t0 = tspan[0] + rng.uniform(-1.0, 1.0, n)
+        p.set_member_tspan(np.vstack([t0, t0 + (tspan[1] - tspan[0]) * rng.uniform(0.2, 1.0, n)]))
+    p.transient()
+    p.shift_x0()
+    p.features()
+    assert p.gpu.get_replay_info()[1]
+    p.assert_bitexact(("F", "xf", "dt", "tf"))
