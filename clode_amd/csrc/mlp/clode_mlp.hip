@@ -1543,18 +1543,24 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
     Tall.alloc(n * 2);
     Call.alloc(n);
     dperm.alloc(n);
-    eloss.alloc(1);
+    eloss.alloc((size_t)epochs);
     upload(h, Xall.p, X, n * 8);
     upload(h, Tall.p, T, n * 8);
     upload(h, Call.p, cls, n * 4);
     h->ensureRows((size_t)batch);
     std::mt19937_64 rng(seed);
     // epoch e's order is a shuffle of epoch e-1's; the next epoch's shuffle
-    // runs on the host while the GPU trains on the current one
-    std::vector<unsigned int> perm(n), next(n);
+    // runs on the host while the GPU trains on the current one.  Every
+    // epoch's order has its own host buffer, untouched after its upload
+    // is queued, and the epoch losses stay on the device until the end, so
+    // the host waits between epochs only for an upload two epochs back.
+    std::vector<unsigned int> perms[2] = {std::vector<unsigned int>(n), std::vector<unsigned int>(n)};
+    hipEvent_t uploaded[2];
+    for (int b = 0; b < 2; ++b)
+        check(hipEventCreateWithFlags(&uploaded[b], hipEventDisableTiming), "hipEventCreate");
     for (size_t i = 0; i < n; ++i)
-        perm[i] = (unsigned int)i;
-    std::shuffle(perm.begin(), perm.end(), rng);
+        perms[0][i] = (unsigned int)i;
+    std::shuffle(perms[0].begin(), perms[0].end(), rng);
     const size_t steps_per_epoch = (n + batch - 1) / batch;
     const double total = (double)steps_per_epoch * epochs;
     const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
@@ -1574,10 +1580,11 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
     for (int l = 0; l < NLAYER; ++l)
         nbias += h->L[l].out;
     long long step = 0;
+    check(hipMemsetAsync(eloss.p, 0, (size_t)epochs * 4, h->stream), "hipMemset");
     check(hipEventRecord(h->ev0, h->stream), "hipEventRecord");
     for (int e = 0; e < epochs; ++e) {
-        upload(h, dperm.p, perm.data(), n * 4);
-        check(hipMemsetAsync(eloss.p, 0, 4, h->stream), "hipMemset");
+        upload(h, dperm.p, perms[e % 2].data(), n * 4);
+        check(hipEventRecord(uploaded[e % 2], h->stream), "hipEventRecord");
         for (size_t s = 0; s < steps_per_epoch; ++s, ++step) {
             const size_t b0 = s * (size_t)batch;
             const int nb = (int)std::min<size_t>((size_t)batch, n - b0);
@@ -1587,7 +1594,7 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
             check(hipGetLastError(), "mlp_gather launch");
             forward(h, nb, true);
             hipLaunchKernelGGL(mlp_loss, dim3((NHEAD * nb + 255) / 256), dim3(256), 0, h->stream, h->Zh.p, nb, h->T.p,
-                               h->C.p, h->ids.p, 1.0f / (float)nb, h->dOut.p, h->dOutT.p, ldT(nb), eloss.p);
+                               h->C.p, h->ids.p, 1.0f / (float)nb, h->dOut.p, h->dOutT.p, ldT(nb), eloss.p + e);
             check(hipGetLastError(), "mlp_loss launch");
             GradSources gs;
             std::memset(&gs, 0, sizeof(gs));
@@ -1617,15 +1624,20 @@ int clode_mlp_train(clode_mlp *h, const float *X, const float *T, const int *cls
             check(hipGetLastError(), "mlp_adam launch");
         }
         if (e + 1 < epochs) {
-            next = perm;  // the buffer being uploaded stays untouched
-            std::shuffle(next.begin(), next.end(), rng);
+            std::vector<unsigned int> &nx = perms[(e + 1) % 2];
+            if (e >= 1)  // epoch e-1's upload read this buffer
+                check(hipEventSynchronize(uploaded[(e + 1) % 2]), "hipEventSynchronize");
+            nx = perms[e % 2];
+            std::shuffle(nx.begin(), nx.end(), rng);
         }
-        float el = 0;
-        download(h, &el, eloss.p, 4);
-        if (epoch_loss)
-            epoch_loss[e] = el / (float)steps_per_epoch;
-        perm.swap(next);
     }
+    for (int b = 0; b < 2; ++b)
+        (void)hipEventDestroy(uploaded[b]);
+    std::vector<float> el((size_t)epochs);
+    download(h, el.data(), eloss.p, (size_t)epochs * 4);
+    if (epoch_loss)
+        for (int e = 0; e < epochs; ++e)
+            epoch_loss[e] = el[(size_t)e] / (float)steps_per_epoch;
     check(hipEventRecord(h->ev1, h->stream), "hipEventRecord");
     check(hipStreamSynchronize(h->stream), "hipStreamSynchronize");
     float ms = 0;
